@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SIMP state-solve path.
+
+Headline: SIMP iterations/s on the 160x80x80 hex8 cantilever (1,024,000
+elements, 3.17M DOF; x=0 face clamped, total -1 z line load on the x=160,
+z=0 edge; volfrac 0.3, rmin 1.5, move 0.2; 4-level Galerkin GMG with
+Chebyshev degree 3; PCG tol 1e-5 relative residual, max_krylov 800), for the
+BF16-MFMA path and the plain FP32 path. A step is one full SIMP iteration:
+filter, projection, moduli, GMG setup, state solve to tol (FP64 true
+residual), compliance, sensitivity, filter transpose, OC bisection. Inputs
+are resident in HBM; the density starts at volfrac, W warm-up iterations are
+run untimed, then K iterations are timed between device synchronisations.
+
+Also reported: matvec GB/s on the 128x64x64 grid (524,288 elements,
+algorithmic bytes 8*ndof + 4*n_elem per product), the live roofline of the
+dominant kernel, and a CPU baseline from the FP64 oracle on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "SIMP iter/s @1M hex8 (BF16-MFMA GMG vs FP32 plain, rtol 1e-5); matvec GB/s"
+C2 = dict(dims=(160, 80, 80), volfrac=0.30, rmin=1.5, move=0.2, n_levels=4, fine_degree=3, coarse_degree=2,
+          tol=1e-5, max_krylov=800)
+C1 = (128, 64, 64)
+HBM_PEAK_GBPS = 8000.0
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool):
+    import torch
+
+    import topogpu_amd as tg
+    from topogpu_amd.bc import generate_bc
+    from topogpu_amd.hip.simp import HipSIMP
+    from topogpu_amd.hip.timing import KernelTimer
+
+    prob = tg.unit_cantilever(C2["dims"], volfrac=C2["volfrac"], rmin=C2["rmin"], load="line")
+    bc = generate_bc(prob.spec)
+    eng = HipSIMP(C2["dims"], bc.free_dofs, prob.force_override, volfrac=C2["volfrac"], rmin=C2["rmin"],
+                  move=C2["move"], precision=precision, n_levels=C2["n_levels"], fine_degree=C2["fine_degree"],
+                  coarse_degree=C2["coarse_degree"], tol=C2["tol"], max_krylov=C2["max_krylov"])
+    for i in range(warmup):
+        r = eng.step()
+        _log(f"{precision} warmup {i + 1}/{warmup}: C={r.compliance:.6g} its={r.iters} conv={r.converged}")
+    timer = KernelTimer() if timed_kernels else None
+    eng.fea.gmg.timer = timer
+    torch.cuda.synchronize()
+    rows = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r = eng.step()
+        rows.append(r)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eng.fea.gmg.timer = None
+    for i, r in enumerate(rows):
+        _log(f"{precision} step {i + 1}/{steps}: C={r.compliance:.6g} its={r.iters} restarts={r.restarts} "
+             f"relres={r.rel_residual:.2e} oc={r.oc_steps}")
+    stats = {
+        "iter_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+        "pcg_iters_mean": float(np.mean([r.iters for r in rows])),
+        "ir_restarts_mean": float(np.mean([r.restarts for r in rows])),
+        "max_rel_residual": float(max(r.rel_residual for r in rows)),
+        "all_converged": bool(all(r.converged for r in rows)),
+        "compliance_last": float(rows[-1].compliance),
+    }
+    kernels = timer.summary() if timer else {}
+    if timed_kernels:
+        for name, rec in saturated_kernel_times(eng, steps).items():
+            kernels.setdefault(name, {}).update(rec)
+    n = {"ndof": eng.fea.n, "n_elem": eng.fea.n_elem}
+    del eng
+    torch.cuda.empty_cache()
+    return stats, kernels, n
+
+
+def saturated_kernel_times(eng, steps: int, reps: int = 50) -> dict:
+    """Average duration of the level-0 kernels on the live operands (E,
+    masks, dinv of the last SIMP iteration), launched back to back between
+    two events so host launch gaps do not enter the measurement."""
+    import torch
+
+    from topogpu_amd.hip import lib as L
+    from topogpu_amd.hip.gmg import KERNEL_NAMES, hex_bytes
+
+    g = eng.fea.gmg
+    lv0 = g.levels[0]
+    n, ne = lv0.n, lv0.n_elem
+    st = torch.cuda.current_stream().cuda_stream
+    u = lv0.da.clone()
+    r, x, d = lv0.r.clone(), lv0.x.clone(), torch.zeros_like(lv0.da)
+    y = torch.zeros_like(lv0.da)
+    dot = torch.zeros(1, dtype=torch.float64, device=u.device)
+    cases = {
+        KERNEL_NAMES[g.levels[0].kind]: (g.op(0), L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1),
+                                         L.TG_EP_CHEB_R, None),
+        KERNEL_NAMES[eng.fea.fine_op().kind]: (eng.fea.fine_op(), L.ep(L.TG_EP_DOT, dot=dot), L.TG_EP_DOT, y),
+    }
+    out = {}
+    for name, (op, ep, mode, yy) in cases.items():
+        for _ in range(3):
+            L.call("tg_apply32", op, L.ptr(u), L.ptr(yy), ep, st)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            L.call("tg_apply32", op, L.ptr(u), L.ptr(yy), ep, st)
+        e.record()
+        torch.cuda.synchronize()
+        us = 1e3 * s.elapsed_time(e) / reps
+        nb = hex_bytes(n, ne, mode)
+        out[name] = {"saturated_avg_us": us, "saturated_epilogue": {L.TG_EP_CHEB_R: "CHEB_R", L.TG_EP_DOT: "DOT"}[mode],
+                     "saturated_bytes_per_launch": nb, "saturated_gbps": nb / (us * 1e-6) / 1e9}
+    return out
+
+
+def matvec_bench(reps: int = 200, warmup: int = 5):
+    """E4 protocol of the reference (run_experiments_e1_e10.py:604-623):
+    5 warm-up products then 200 timed, here per precision, with the
+    relative error against the FP64 GPU operator on the same inputs."""
+    import torch
+
+    from topogpu_amd import fem
+    from topogpu_amd.hip import lib as L
+
+    dims = C1
+    dev = torch.device("cuda", 0)
+    L.init_device(fem.unit_stiffness(), 0)
+    n_nodes = int(np.prod([d + 1 for d in dims]))
+    ndof, ne = 3 * n_nodes, int(np.prod(dims))
+    mask = np.full(n_nodes, 7, np.uint8)
+    mask.reshape(dims[0] + 1, -1)[0] = 0                         # x=0 face clamped
+    rho = np.random.default_rng(0).uniform(1e-3, 1.0, ne)
+    E = rho ** 3
+    u = np.random.default_rng(1).standard_normal(ndof)
+    free = np.repeat(mask, 3).reshape(-1, 3) >> np.arange(3) & 1
+    u = u * free.ravel()
+    dmask = torch.from_numpy(mask).to(dev)
+    E64, u64 = torch.from_numpy(E).to(dev), torch.from_numpy(u).to(dev)
+    E32, u32 = E64.float(), u64.float()
+    y64 = torch.zeros(ndof, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    L.call("tg_apply64", L.grid(dims), L.ptr(dmask), L.ptr(E64), L.ptr(u64), None, L.ptr(y64), None, None, st)
+    ref = y64.cpu().numpy()
+    out = {"grid": "x".join(map(str, dims)), "n_elem": ne, "ndof": ndof,
+           "bytes_per_matvec": 8 * ndof + 4 * ne, "flops_per_matvec": 1152 * ne}
+    kinds = {"fp32": L.TG_OP_HEX_F32, "bf16": L.TG_OP_HEX_BF16, "bf16x3": L.TG_OP_HEX_BF16X3}
+    y = torch.zeros(ndof, dtype=torch.float32, device=dev)
+    for name, kind in kinds.items():
+        op = L.TgOp32(kind, L.grid(dims), L.ptr(dmask), L.ptr(E32), None)
+        ep = L.ep(L.TG_EP_STORE)
+        for _ in range(warmup):
+            L.call("tg_apply32", op, L.ptr(u32), L.ptr(y), ep, st)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            L.call("tg_apply32", op, L.ptr(u32), L.ptr(y), ep, st)
+        e.record()
+        torch.cuda.synchronize()
+        us = 1e3 * s.elapsed_time(e) / reps
+        got = y.cpu().numpy().astype(np.float64)
+        out[name] = {"us": us, "gbps": out["bytes_per_matvec"] / (us * 1e-6) / 1e9,
+                     "tflops": out["flops_per_matvec"] / (us * 1e-6) / 1e12,
+                     "rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                     "rel_inf_vs_fp64": float(np.abs(got - ref).max() / np.abs(ref).max())}
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(20):
+        L.call("tg_apply64", L.grid(dims), L.ptr(dmask), L.ptr(E64), L.ptr(u64), None, L.ptr(y64), None, None, st)
+    e.record()
+    torch.cuda.synchronize()
+    us = 1e3 * s.elapsed_time(e) / 20
+    out["fp64"] = {"us": us, "gbps": (16 * ndof + 8 * ne) / (us * 1e-6) / 1e9}
+    return out
+
+
+def cpu_baseline(fine_ops_per_iter: float, sample_matvecs: int = 2):
+    """FP64 oracle on the host: the oracle's matrix-free operator (gather,
+    GEMM with KE, bincount scatter -- MatrixFreeKff restated) timed at
+    160x80x80, plus the host filter / sensitivity / OC pass of one SIMP
+    iteration at the same size. iter/s = 1 / (fine_ops_per_iter * t_matvec +
+    t_simp_host), with fine_ops_per_iter counted by the GPU run of the same
+    algorithm (coarse levels and the coarsest solve are not charged)."""
+    import oracle
+    from oracle import simp as osimp
+
+    try:
+        from threadpoolctl import threadpool_info
+        blas_threads = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        blas_threads = 1
+    dims = C2["dims"]
+    ne = int(np.prod(dims))
+    ndof = 3 * int(np.prod([d + 1 for d in dims]))
+    edof = oracle.edof_table_3d(*dims)
+    rng = np.random.default_rng(0)
+    E = rng.uniform(1e-3, 1.0, ne) ** 3
+    fm = np.ones(ndof, bool)
+    fm[: 3 * 81 * 81] = False
+    u = rng.standard_normal(ndof)
+    osimp.matfree_matvec(u, E, edof, fm)                        # warm
+    t0 = time.perf_counter()
+    for _ in range(sample_matvecs):
+        osimp.matfree_matvec(u, E, edof, fm)
+    t_mv = (time.perf_counter() - t0) / sample_matvecs
+    t0 = time.perf_counter()
+    H = oracle.build_density_filter(*dims, C2["rmin"])
+    t_filter_build = time.perf_counter() - t0
+    rho = np.full(ne, C2["volfrac"])
+    t0 = time.perf_counter()
+    rf = H @ rho
+    rp = oracle.heaviside(rf, 1.0)
+    dh = oracle.heaviside_deriv(rf, 1.0)
+    ce = oracle.element_compliance(u, edof)
+    dc = H.T @ (-3.0 * rp ** 2 * ce * dh)
+    dv = H.T @ dh
+    oracle.oc_update_filtered(rho, dc, dv, C2["volfrac"], C2["move"], H, 1.0, 0.5, True)
+    t_host = time.perf_counter() - t0
+    per_iter = fine_ops_per_iter * t_mv + t_host
+    return {"value": 1.0 / per_iter, "unit": "iter/s", "cores": int(blas_threads), "kind": "port",
+            "sample": (f"FP64 numpy oracle at 160x80x80: {sample_matvecs} matrix-free matvecs "
+                       f"({t_mv:.3f} s each; gather/bincount single-threaded, BLAS {blas_threads} threads) "
+                       f"x {fine_ops_per_iter:.1f} fine-operator applications per SIMP iteration (counted on "
+                       f"the GPU run) + one filter/sensitivity/OC pass ({t_host:.2f} s; filter build "
+                       f"{t_filter_build:.2f} s not charged)"),
+            "t_matvec_s": t_mv, "t_simp_host_s": t_host}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="bf16")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-matvec", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this path has no multi-GPU mode (see DESIGN.md for how it would "
+              "shard); run with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 2
+
+    bf, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, timed_kernels=True)
+    _log(f"{args.precision}: {bf['iter_per_s']:.3f} iter/s")
+    fp = None
+    if not args.no_fp32:
+        fp, _, _ = simp_bench("fp32", args.steps, args.warmup, timed_kernels=False)
+        _log(f"fp32: {fp['iter_per_s']:.3f} iter/s")
+    mv = None if args.no_matvec else matvec_bench()
+    # dominant level-0 kernel: most device time per SIMP step (launch count
+    # from the in-loop timer x saturated per-launch duration)
+    timed = {k: v for k, v in kernels.items() if "saturated_avg_us" in v and "launches" in v}
+    dom_name, dom = max(timed.items(), key=lambda kv: kv[1]["saturated_avg_us"] * kv[1]["launches"])
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["saturated_gbps"], "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": dom["saturated_gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                "avg_us": dom["saturated_avg_us"], "epilogue": dom["saturated_epilogue"],
+                "launches_per_step": dom["launches"] / args.steps,
+                "algorithmic_bytes_per_launch": dom["saturated_bytes_per_launch"]}
+    fine_ops = sum(k["launches"] for k in kernels.values()) / args.steps
+    cpu = None if args.no_cpu else cpu_baseline(fine_ops)
+    line = {
+        "metric": METRIC, "value": bf["iter_per_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": bf["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16" if args.precision.startswith("bf16") else "fp32",
+        "data": "synthetic",
+        "config": {"workload": "SIMP cantilever 160x80x80 hex8 (1,024,000 elem), line load, 4-level Galerkin "
+                               "GMG, Chebyshev degree 3, PCG tol 1e-5",
+                   "precision": args.precision, **{k: v for k, v in C2.items() if k != "dims"},
+                   "grid": "160x80x80", "ndof": sizes["ndof"], "n_elem": sizes["n_elem"]},
+        "solve": {k: bf[k] for k in ("pcg_iters_mean", "ir_restarts_mean", "max_rel_residual", "all_converged",
+                                     "compliance_last")},
+        "fp32_plain": fp, "speedup_vs_fp32": (bf["iter_per_s"] / fp["iter_per_s"]) if fp else None,
+        "matvec": mv, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

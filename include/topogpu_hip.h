@@ -1,0 +1,258 @@
+/* topogpu_hip.h -- C ABI of the MI355X (gfx950) SIMP state-solve library.
+ *
+ * Plain C: device pointers, sizes and an opaque stream handle (a hipStream_t
+ * passed as void*; NULL = the default stream). Every entry returns 0 on
+ * success or a TG_ERR_* code; tg_last_error() then describes the failure.
+ * All work is enqueued on the given stream; nothing is host-synchronous
+ * unless documented. Device buffers are owned by the caller; the library
+ * never frees them. Reductions write their results to caller-provided device
+ * scalars so that iterative solvers keep their scalars on the GPU.
+ *
+ * The library replaces the device side of the reference's SolverV4 path
+ * (nbbllxx0/TopoGPU). Each entry cites the reference interface it replaces.
+ * Reductions use one library-owned workspace per device: calls that reduce
+ * (a non-NULL dot/norm output) must not run concurrently on one device.
+ *
+ * Vector layout: full length 3*(nelx+1)*(nely+1)*(nelz+1), node (i,j,k) at
+ * (i*(nely+1)+j)*(nelz+1)+k, DOF = 3*node + component (reference
+ * bc_generator.py:104-105). A per-node byte mask marks free DOFs (bit c set =
+ * component c free); operators ignore fixed input DOFs and write zero to
+ * fixed output DOFs, i.e. they apply M K M, the embedding of K_ff.
+ */
+#ifndef TOPOGPU_HIP_H
+#define TOPOGPU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TG_OK = 0,
+  TG_ERR_ARG = 1,     /* invalid argument or unsupported shape            */
+  TG_ERR_HIP = 2,     /* HIP runtime / kernel launch failure              */
+  TG_ERR_STATE = 3,   /* tg_init not called / library state missing       */
+  TG_ERR_NOT_SPD = 4  /* dense Cholesky failed                            */
+};
+
+typedef struct {
+  int32_t nelx, nely, nelz; /* elements per axis (>= 1) */
+} tg_grid;
+
+/* ---- library state --------------------------------------------------- */
+
+/* Upload the 24x24 unit element stiffness (host, row-major, FP64) to device
+ * constant memory of the current device, derive its FP32/BF16 forms and the
+ * eight Galerkin child matrices, and allocate the reduction workspace.
+ * Replaces the per-solver uploads of GPUFEMSolver._upload_cupy
+ * (src/gpu_fem/fem_gpu.py:361-368) and FusedMatvec.__init__ KE packing
+ * (src/gpu_fem/cuda_fused_matvec.py:279-292, 316-330). */
+int tg_init(const double* ke_unit_host);
+const char* tg_last_error(void);
+int tg_version(void);
+
+/* ---- operators --------------------------------------------------------- */
+
+enum {
+  /* level-0 hex8 operator, weights = E per element (FP32), vector FMA      */
+  TG_OP_HEX_F32 = 1,
+  /* level-0 hex8 operator, element GEMM on BF16 MFMA, FP32 accumulate       */
+  TG_OP_HEX_BF16 = 2,
+  /* as TG_OP_HEX_BF16 with hi/lo split operands (3 MFMA passes)            */
+  TG_OP_HEX_BF16X3 = 3,
+  /* Galerkin level-1 operator of a fine hex grid with even dims: element
+   * matrix sum_s E_child(s) * Q_s, weights = fine E (FP32)                 */
+  TG_OP_Q8_F32 = 4,
+  /* stored element matrices (576 FP32 per element, row-major)               */
+  TG_OP_STORED_F32 = 5
+};
+
+typedef struct {
+  int32_t kind;               /* TG_OP_*                                     */
+  tg_grid grid;               /* grid the operator acts on                   */
+  const uint8_t* mask;        /* per-node free mask of that grid             */
+  const float* weights;       /* HEX*: E[elem]; Q8: fine E[fine elem]        */
+  const float* elem_mats;     /* STORED: 576 per element                     */
+} tg_op32;
+
+enum {
+  TG_EP_STORE = 0,      /* y = A u                                           */
+  TG_EP_DOT = 1,        /* y = A u ;            *dot = sum(u*y)             */
+  TG_EP_RESID = 2,      /* y = rhs - A u                                     */
+  TG_EP_RESID_NORM = 3, /* y = rhs - A u ;      *dot = sum(y*y)             */
+  TG_EP_CHEB = 4,       /* t = dinv*(rhs - A u); d = a1*t + a2*d;
+                           x = u + d   (x is output only; x != u)           */
+  TG_EP_CHEB_R = 5,     /* residual-update Chebyshev step on a correction u:
+                           r -= A u; d = a1*dinv*r + a2*u; x += a3*u + d
+                           (r, x in/out; d output, d != u)                  */
+  TG_EP_RESID_UPDATE = 6 /* r -= A u (r in/out)                             */
+};
+
+typedef struct {
+  int32_t mode;         /* TG_EP_*                                           */
+  const float* rhs;
+  const float* dinv;
+  float* d;             /* CHEB: in/out (not read when a2 == 0); CHEB_R: out */
+  float* x;             /* CHEB: output iterate; CHEB_R: in/out iterate     */
+  float* r;             /* CHEB_R / RESID_UPDATE: residual in/out           */
+  float a1, a2, a3;     /* CHEB / CHEB_R coefficients                       */
+  double* dot;          /* device scalar for DOT / RESID_NORM               */
+} tg_ep32;
+
+/* y (or the epilogue outputs) = A u for a single-precision operator.
+ * Replaces FusedMatvec.matvec / _launch (cuda_fused_matvec.py:336-409),
+ * MatrixFreeKff.matvec in FP32 (solver_v2.py:1183-1223), the coarse CSR SpMV
+ * (multigrid_v4.py:948-1001) and the elementwise work fused around them in
+ * _chebyshev_steps (multigrid_v4.py:860-889). */
+int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_ep32* ep, void* stream);
+
+/* Diagnostic: one v_mfma_f32_16x16x32_bf16 on one wavefront. a, b: 64 lanes
+ * x 8 BF16 bit patterns (device), d: 64 lanes x 4 floats. Used by the tests
+ * to confirm the operand layout the element kernel assumes. No reference
+ * counterpart (the reference's WMMA fragments are opaque). */
+int tg_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, void* stream);
+
+/* FP64 level-0 operator: y = M K(E) M u, or with rhs != NULL the residual
+ * y = rhs - A u; if y32 != NULL also writes float(y); if sumsq != NULL writes
+ * sum(y*y) (FP64). Replaces MatrixFreeKff.matvec in FP64
+ * (solver_v2.py:1183-1223) as used for the outer operator and true residual
+ * (solver_v4.py:177-178, multigrid_v4.py:994). */
+int tg_apply64(const tg_grid* g, const uint8_t* mask, const double* E, const double* u,
+               const double* rhs, double* y, float* y32, double* sumsq, void* stream);
+
+/* Inverse diagonal of M K(E) M (fixed DOFs -> 1). Replaces
+ * MatrixFreeKff.extract_diagonal (solver_v2.py:1229-1252) and the D^-1 setup
+ * (multigrid_v4.py:684-686). dinv64 or dinv32 may be NULL. */
+int tg_diag_inv_hex(const tg_grid* g, const uint8_t* mask, const double* E, double* dinv64,
+                    float* dinv32, void* stream);
+/* Same for stored element matrices (FP64) of a coarse level
+ * (assemble_level1_galerkin diagonal, multigrid_v4.py:268-269, 751-752). */
+int tg_diag_inv_stored(const tg_grid* g, const uint8_t* mask, const double* mats64, float* dinv32,
+                       void* stream);
+/* Same for the implicit level-1 operator (weights = fine E, FP64). */
+int tg_diag_inv_q8(const tg_grid* coarse, const uint8_t* mask, const double* fine_E,
+                   float* dinv32, void* stream);
+
+/* ---- multigrid ---------------------------------------------------------- */
+
+/* rc = M_c R rf with R = P^T, P the row-normalised trilinear prolongation
+ * of _build_scalar_prolongation (solver_v2.py:62-149) kron I3, coarse grid
+ * max(1, n//2) per axis (multigrid_v4.py:525-529). Replaces R_gpu @ residual
+ * (multigrid_v4.py:999). */
+int tg_restrict32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c,
+                  const float* rf, float* rc, void* stream);
+/* xf = M_f P ec (accumulate = 0) or xf += M_f P ec (accumulate = 1).
+ * Replaces P_gpu @ e_c (multigrid_v4.py:1001). */
+int tg_prolong32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f,
+                 const float* ec, float* xf, int accumulate, void* stream);
+
+/* Galerkin coarse element matrices (FP64, 576 per coarse element):
+ *   K_C = sum_{children f} (M_f P_f)^T K_f (M_f P_f)
+ * with P_f the local 24x24 restriction of the prolongation above to child f
+ * and M_f its free mask (use_mask = 0 skips the mask). K_f is E[f]*KE when
+ * fine_mats == NULL (fine_E used), else fine_mats[f]. coarse_mats32 (may be
+ * NULL) receives an FP32 copy. Replaces assemble_level1_galerkin and the
+ * SpGEMM R@(K@P) (multigrid_v4.py:248-270, 747-752). */
+int tg_galerkin_elems(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f,
+                      int use_mask, const double* fine_E, const double* fine_mats,
+                      double* coarse_mats64, float* coarse_mats32, void* stream);
+
+/* Dense assembly of the free-DOF matrix of a level from element matrices:
+ * A[i][j] (n_free x n_free, row-major FP64), free_index[dof] = compact index
+ * or -1. Replaces K_coarsest.toarray() (multigrid_v4.py:781). */
+int tg_assemble_dense(const tg_grid* g, const double* mats64, const int32_t* free_index,
+                      int32_t n_free, double* A, void* stream);
+
+/* In-place dense FP64 Cholesky of A (n x n, row-major SPD) followed by the
+ * explicit inverse A^-1 (full symmetric) written to A. Host-synchronous.
+ * Replaces cp.linalg.cholesky + solve_triangular (multigrid_v4.py:781-801,
+ * 977-978). Returns TG_ERR_NOT_SPD when the factorisation fails. */
+int tg_dense_spd_inverse(double* A, int32_t n, void* stream);
+
+/* ec = scatter(Ainv * gather(rc)) over the free DOFs of the coarsest level;
+ * fixed DOFs of ec are written as 0. free_list[i] = full DOF of compact i.
+ * Replaces _solve_coarsest (multigrid_v4.py:969-984). */
+int tg_coarsest_apply(const double* Ainv, int32_t n_free, const int32_t* free_list,
+                      int64_t n_full, const float* rc, float* ec, void* stream);
+
+/* ---- Krylov vector kernels (FP32 vectors, FP64 reductions) -------------- */
+
+/* *out = sum(a*b) (and *out2 = sum(a*c) when c != NULL), FP64 accumulate.
+ * Replaces cp.dot host-synced reductions (solver_v2.py:588-619). */
+int tg_dot32(int64_t n, const float* a, const float* b, const float* c, double* out,
+             double* out2, void* stream);
+int tg_dot64(int64_t n, const double* a, const double* b, double* out, void* stream);
+/* PCG step: alpha = *rz / *pq; x += alpha p; r -= alpha q; *rr = sum(r*r).
+ * Replaces the vector updates of _cupy_pcg (solver_v2.py:606-611). */
+int tg_pcg_xr(int64_t n, float* x, float* r, const float* p, const float* q, const double* rz,
+              const double* pq, double* rr, void* stream);
+/* Flexible (Polak-Ribiere) direction: beta = -(*zq) / (*pq); p = z + beta p.
+ * beta = z_{k+1}.(r_{k+1}-r_k)/(z_k.r_k) rewritten with r_{k+1}-r_k = -alpha q. */
+int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const double* pq,
+               void* stream);
+/* x64 += double(d32): iterative-refinement update of the FP64 iterate. */
+int tg_axpy_f32_to_f64(int64_t n, double* x64, const float* d32, void* stream);
+/* First Chebyshev step from a zero iterate: d = x = a * dinv * rhs and, when
+ * r != NULL, r = rhs (_chebyshev_steps, multigrid_v4.py:874-879, A x0 = 0). */
+int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, float* d, float* x,
+                   float* r, void* stream);
+/* Power-iteration step for lambda_max(D^-1 K): *vdv = sum(v*v/dinv), then
+ * v <- dinv * kv (kv = K v). The Rayleigh quotient (v.Kv)/(v.Dv) is
+ * scale-free, so v is not renormalised (_estimate_fine_lambda_max,
+ * multigrid_v4.py:273-295). */
+int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv,
+                    void* stream);
+/* Fill helpers (graph-safe replacements for memset). */
+int tg_fill32(int64_t n, float* x, float value, void* stream);
+int tg_fill64(int64_t n, double* x, double value, void* stream);
+
+/* ---- SIMP density / sensitivity / OC (FP64) ----------------------------- */
+
+/* out[i] = (sum_k w_k * s(in)[i+off_k]) * (out_scale ? out_scale[i] : 1),
+ * s(in)[j] = in[j] * (in_scale ? in_scale[j] : 1); offsets (dx,dy,dz)
+ * outside the grid are skipped. With in_scale = NULL and out_scale = 1/Hs
+ * this is the density filter H (pub_simp_solver.py:215-243, H_mat.dot); with
+ * in_scale = 1/Hs and out_scale = NULL it is H^T (:368, :371). */
+int tg_stencil64(const tg_grid* g, int32_t m, const int32_t* offs_host, const double* w_host,
+                 const double* in, const double* in_scale, const double* out_scale,
+                 double* out, void* stream);
+
+/* rho_phys = heaviside(rho_f, beta, eta) (identity if beta < 1e-6),
+ * dh = d rho_phys / d rho_f, E = Emin + (E0-Emin) rho_phys^penal (FP64 and
+ * FP32). pub_simp_solver.py:245-253, 347-350. Any output may be NULL. */
+int tg_density_chain(int64_t n, const double* rho_f, double beta, double eta, double penal,
+                     double emin, double e0, double* rho_phys, double* dh, double* E64,
+                     float* E32, void* stream);
+
+/* Element sensitivity: ce = u_e^T KE u_e from the FP64 full displacement,
+ * dcf = -penal (E0-Emin) rho_phys^(penal-1) ce * dh (dh may be NULL = 1).
+ * pub_simp_solver.py:361-367, solver_v4.py:246-251. ce may be NULL. */
+int tg_sensitivity64(const tg_grid* g, const double* U, const double* rho_phys,
+                     const double* dh, double penal, double e_range, double* ce, double* dcf,
+                     void* stream);
+
+/* OC bisection of pub_simp_solver.py:319-340 on the device.
+ * oc_prepare: q = rho*sqrt(-min(dc,-1e-12)/max(dv,1e-12)),
+ *   lo = max(rho-move, 1e-3), hi = min(rho+move, 1) ; state <- {l1=0, l2=1e9}.
+ * oc_bisect_step: one bisection step (no-op once converged): lm=(l1+l2)/2,
+ *   rn = clip(q/sqrt(lm), lo, hi), volume = sum(heaviside(H rn))/n, update
+ *   the bracket, converged when l2-l1 < 1e-9. Filter as in tg_stencil64.
+ * oc_finalize: rho_new = clip(q/sqrt(lm_last)); *change = max|rho_new-rho|.
+ * state: 8 device doubles {l1, l2, lm, done, steps, vol, -, -}. */
+int tg_oc_prepare(int64_t n, const double* rho, const double* dc, const double* dv, double move,
+                  double* q, double* lo, double* hi, double* state, void* stream);
+int tg_oc_bisect_step(const tg_grid* g, int32_t m, const int32_t* offs_host,
+                      const double* w_host, const double* hs_inv, const double* q,
+                      const double* lo, const double* hi, double beta, double eta,
+                      double volfrac, int use_heaviside, double* state, void* stream);
+int tg_oc_finalize(int64_t n, const double* rho, const double* q, const double* lo,
+                   const double* hi, const double* state, double* rho_new, double* change,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPOGPU_HIP_H */

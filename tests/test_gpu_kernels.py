@@ -1,0 +1,395 @@
+"""Kernel-level parity of libtopohip against the FP64 oracle (GPU).
+
+Tolerances: FP64 kernels 1e-12 relative; FP32 vector kernels 2e-6; BF16
+MFMA 1.5e-2 (inputs and KE rounded to 8-bit mantissas, FP32 accumulate);
+split BF16 (hi/lo, 3 MFMA) 1e-4. Inputs of the single-precision kernels are
+rounded to FP32 first so that only kernel arithmetic is measured.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from oracle import gmg as ogmg
+from oracle import simp as osimp
+from topogpu_amd import fem
+from topogpu_amd.hip import lib as L
+
+from gpu_util import DEV, cantilever_bc, free_mask_full, h, init, rel_inf, rel_l2, stream, t
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _init(gpu):
+    init()
+
+
+def _bf16_bits(x: np.ndarray) -> np.ndarray:
+    return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def test_mfma_operand_layout():
+    rng = np.random.default_rng(0)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    lanes = np.arange(64)
+    a = np.zeros((64, 8), np.float32)
+    b = np.zeros((64, 8), np.float32)
+    for l in lanes:
+        for v in range(8):
+            a[l, v] = A[l % 16, 8 * (l // 16) + v]
+            b[l, v] = B[8 * (l // 16) + v, l % 16]
+    da = t(_bf16_bits(a).ravel())
+    db = t(_bf16_bits(b).ravel())
+    dd = torch.zeros(256, dtype=torch.float32, device=DEV)
+    L.call("tg_mfma_probe", L.ptr(da), L.ptr(db), L.ptr(dd), stream())
+    d = h(dd).reshape(64, 4)
+    ref = A @ B
+    got = np.zeros((16, 16), np.float32)
+    for l in lanes:
+        for v in range(4):
+            got[4 * (l // 16) + v, l % 16] = d[l, v]
+    assert np.array_equal(got, ref)
+
+
+def _setup(dims, seed=0, point_support=False):
+    free, F, ndof = cantilever_bc(dims, point_support)
+    rng = np.random.default_rng(seed)
+    n_el = int(np.prod(dims))
+    E = rng.uniform(1e-3, 1.0, n_el) ** 3
+    nn = ndof // 3
+    mask = fem.node_mask_from_free(free, nn)
+    return free, F, ndof, E, mask, rng
+
+
+OPS = {"fp32": (L.TG_OP_HEX_F32, 2e-6), "bf16": (L.TG_OP_HEX_BF16, 1.5e-2), "bf16x3": (L.TG_OP_HEX_BF16X3, 1e-4)}
+
+
+@pytest.mark.parametrize("dims", [(5, 3, 4), (9, 7, 17), (16, 9, 31), (4, 8, 15), (2, 2, 2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "bf16x3"])
+def test_hex_operator_matches_oracle(dims, prec):
+    free, F, ndof, E, mask, rng = _setup(dims, 1, point_support=True)
+    edof = oracle.edof_table_3d(*dims)
+    fm = free_mask_full(free, ndof)
+    u = rng.standard_normal(ndof)
+    if prec == "fp64":
+        y = torch.zeros(ndof, dtype=torch.float64, device=DEV)
+        L.call("tg_apply64", L.grid(dims), L.ptr(t(mask)), L.ptr(t(E)), L.ptr(t(u)), None, L.ptr(y), None, None,
+               stream())
+        ref = osimp.matfree_matvec(u, E, edof, fm)
+        assert rel_l2(h(y), ref) < 1e-13
+        return
+    kind, tol = OPS[prec]
+    E32, u32 = E.astype(np.float32), u.astype(np.float32)
+    ref = osimp.matfree_matvec(u32.astype(np.float64), E32.astype(np.float64), edof, fm)
+    y = torch.full((ndof,), np.nan, dtype=torch.float32, device=DEV)
+    dE = t(E32)
+    op = L.TgOp32(kind, L.grid(dims), L.ptr(t(mask)), L.ptr(dE), None)
+    ep = L.ep(L.TG_EP_STORE)
+    L.call("tg_apply32", op, L.ptr(t(u32)), L.ptr(y), ep, stream())
+    got = h(y)
+    assert np.all(got[~fm] == 0.0)
+    assert rel_l2(got, ref) < tol, (prec, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3", "fp32"])
+def test_rigid_translation_is_exact_zero(prec):
+    """KE annihilates translations; the MFMA kernel removes each element's
+    first-node displacement before rounding, so a translation gives 0."""
+    dims = (9, 7, 17)
+    ndof = 3 * 10 * 8 * 18
+    mask = np.full(ndof // 3, 7, np.uint8)
+    E = np.random.default_rng(3).uniform(0.1, 1.0, int(np.prod(dims))).astype(np.float32)
+    u = np.tile(np.array([0.3, -1.7, 2.5], np.float32), ndof // 3)
+    y = torch.zeros(ndof, dtype=torch.float32, device=DEV)
+    op = L.TgOp32(OPS[prec][0], L.grid(dims), L.ptr(t(mask)), L.ptr(t(E)), None)
+    ep = L.ep(L.TG_EP_STORE)
+    L.call("tg_apply32", op, L.ptr(t(u)), L.ptr(y), ep, stream())
+    got = h(y)
+    if prec == "fp32":
+        assert np.abs(got).max() < 1e-5
+    else:
+        assert np.abs(got).max() == 0.0
+
+
+@pytest.mark.parametrize("kind", [L.TG_OP_HEX_F32, L.TG_OP_HEX_BF16])
+def test_epilogues(kind):
+    dims = (6, 5, 7)
+    free, F, ndof, E, mask, rng = _setup(dims, 2)
+    edof = oracle.edof_table_3d(*dims)
+    fm = free_mask_full(free, ndof)
+    E32 = E.astype(np.float32)
+    u = (rng.standard_normal(ndof) * fm).astype(np.float32)
+    rhs = (rng.standard_normal(ndof) * fm).astype(np.float32)
+    dinv = rng.uniform(0.5, 2.0, ndof).astype(np.float32)
+    d0 = (rng.standard_normal(ndof) * fm).astype(np.float32)
+    Au = osimp.matfree_matvec(u.astype(float), E32.astype(float), edof, fm)
+    tol = 2e-6 if kind == L.TG_OP_HEX_F32 else 1.5e-2
+    op = L.TgOp32(kind, L.grid(dims), L.ptr(t(mask)), L.ptr(t(E32)), None)
+    dot = torch.zeros(2, dtype=torch.float64, device=DEV)
+    y = torch.zeros(ndof, dtype=torch.float32, device=DEV)
+    du, drhs = t(u), t(rhs)
+    # DOT
+    L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), stream())
+    assert h(dot)[0] == pytest.approx(float(u.astype(float) @ Au), rel=tol)
+    # RESID_NORM
+    L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_RESID_NORM, rhs=drhs, dot=dot), stream())
+    r = np.where(fm, rhs - Au, 0.0)
+    assert rel_l2(h(y), r) < tol
+    assert h(dot)[0] == pytest.approx(float(r @ r), rel=tol)
+    # CHEB
+    dd, xo, ddinv = t(d0), torch.zeros(ndof, dtype=torch.float32, device=DEV), t(dinv)
+    a1, a2 = 0.7, 0.2
+    L.call("tg_apply32", op, L.ptr(du), None, L.ep(L.TG_EP_CHEB, rhs=drhs, dinv=ddinv, d=dd, x=xo, a1=a1, a2=a2),
+           stream())
+    dn = np.where(fm, a1 * dinv * (rhs - Au) + a2 * d0, 0.0)
+    assert rel_l2(h(dd), dn) < tol
+    assert rel_l2(h(xo), np.where(fm, u + dn, 0.0)) < tol
+    # CHEB_R: r -= A u ; d = a1 dinv r + a2 u ; x += a3 u + d
+    x0 = (rng.standard_normal(ndof) * fm).astype(np.float32)
+    dr, dx, dout = t(rhs), t(x0), torch.zeros(ndof, dtype=torch.float32, device=DEV)
+    L.call("tg_apply32", op, L.ptr(du), None,
+           L.ep(L.TG_EP_CHEB_R, dinv=ddinv, d=dout, x=dx, r=dr, a1=a1, a2=a2, a3=1.0), stream())
+    rn = np.where(fm, rhs - Au, 0.0)
+    dn = np.where(fm, a1 * dinv * rn + a2 * u, 0.0)
+    assert rel_l2(h(dr), rn) < tol
+    assert rel_l2(h(dout), dn) < tol
+    assert rel_l2(h(dx), np.where(fm, x0 + u + dn, 0.0)) < tol
+    # RESID_UPDATE
+    dr = t(rhs)
+    L.call("tg_apply32", op, L.ptr(du), None, L.ep(L.TG_EP_RESID_UPDATE, r=dr), stream())
+    assert rel_l2(h(dr), rn) < tol
+
+
+def test_diag_inv_hex():
+    dims = (7, 4, 5)
+    free, F, ndof, E, mask, rng = _setup(dims, 4, point_support=True)
+    edof = oracle.edof_table_3d(*dims)
+    fm = free_mask_full(free, ndof)
+    d = osimp.matfree_diag(E, edof, fm, ndof)
+    d64 = torch.zeros(ndof, dtype=torch.float64, device=DEV)
+    d32 = torch.zeros(ndof, dtype=torch.float32, device=DEV)
+    L.call("tg_diag_inv_hex", L.grid(dims), L.ptr(t(mask)), L.ptr(t(E)), L.ptr(d64), L.ptr(d32), stream())
+    assert rel_inf(h(d64), 1.0 / d) < 1e-14
+    assert rel_inf(h(d32), 1.0 / d) < 1e-6
+
+
+@pytest.mark.parametrize("dims_f", [(8, 6, 4), (7, 5, 3), (3, 2, 1), (6, 2, 1)])
+def test_transfers_match_oracle_prolongation(dims_f):
+    dims_c = fem.coarse_dims(dims_f)
+    free, F, ndof, E, mask_f, rng = _setup(dims_f, 5, point_support=dims_f[2] > 1)
+    mask_c = fem.injection_mask(dims_f, dims_c, mask_f)
+    P = sp.kron(ogmg.scalar_prolongation(dims_f, dims_c), sp.eye(3), format="csr")
+    nfd, ncd = P.shape
+    fm = np.repeat(mask_f, 3).reshape(-1, 3) >> np.arange(3) & 1
+    cm = np.repeat(mask_c, 3).reshape(-1, 3) >> np.arange(3) & 1
+    fm, cm = fm.ravel().astype(bool), cm.ravel().astype(bool)
+    ec = (rng.standard_normal(ncd) * cm).astype(np.float32)
+    xf0 = (rng.standard_normal(nfd) * fm).astype(np.float32)
+    xf = t(xf0)
+    L.call("tg_prolong32", L.grid(dims_f), L.grid(dims_c), L.ptr(t(mask_f)), L.ptr(t(ec)), L.ptr(xf), 1, stream())
+    ref = xf0 + np.where(fm, P @ ec.astype(float), 0.0)
+    assert rel_l2(h(xf), ref) < 1e-6
+    xf = t(xf0)
+    L.call("tg_prolong32", L.grid(dims_f), L.grid(dims_c), L.ptr(t(mask_f)), L.ptr(t(ec)), L.ptr(xf), 0, stream())
+    assert rel_l2(h(xf), np.where(fm, P @ ec.astype(float), 0.0)) < 1e-6
+    rf = (rng.standard_normal(nfd) * fm).astype(np.float32)
+    rc = torch.zeros(ncd, dtype=torch.float32, device=DEV)
+    L.call("tg_restrict32", L.grid(dims_f), L.grid(dims_c), L.ptr(t(mask_c)), L.ptr(t(rf)), L.ptr(rc), stream())
+    assert rel_l2(h(rc), np.where(cm, P.T @ rf.astype(float), 0.0)) < 1e-6
+
+
+def test_injection_mask_matches_oracle():
+    for dims in [(8, 6, 4), (7, 5, 3), (12, 6, 3)]:
+        free, F, ndof, E, mask, rng = _setup(dims, 0)
+        dc = fem.coarse_dims(dims)
+        mine = fem.free_from_node_mask(fem.injection_mask(dims, dc, mask))
+        assert np.array_equal(mine, ogmg.coarse_free_injection(dims, dc, free))
+
+
+def _oracle_levels(dims, free, E, n_levels):
+    edof = oracle.edof_table_3d(*dims)
+    ndof = 3 * int(np.prod([d + 1 for d in dims]))
+    Kff = oracle.assemble_k(edof, E, ndof)[free][:, free]
+    return ogmg.galerkin_coarse_operators(Kff, ogmg.level_dims(*dims, n_levels), free)
+
+
+def _apply_stored(dims, mask, mats32, u):
+    y = torch.zeros(u.size, dtype=torch.float32, device=DEV)
+    op = L.TgOp32(L.TG_OP_STORED_F32, L.grid(dims), L.ptr(t(mask)), None, L.ptr(mats32))
+    L.call("tg_apply32", op, L.ptr(t(u.astype(np.float32))), L.ptr(y),
+           L.ep(L.TG_EP_STORE), stream())
+    return h(y)
+
+
+@pytest.mark.parametrize("dims,point", [((8, 4, 4), False), ((8, 4, 4), True), ((7, 5, 6), False),
+                                        ((12, 6, 3), False)])
+def test_galerkin_levels_match_ptkp(dims, point):
+    free, F, ndof, E, mask0, rng = _setup(dims, 6, point_support=point)
+    levels = _oracle_levels(dims, free, E, 3)
+    ldims = fem.level_dims(dims, 3)
+    masks = [mask0]
+    for lv in (1, 2):
+        masks.append(fem.injection_mask(ldims[lv - 1], ldims[lv], masks[-1]))
+    fine_m = None
+    for lv in (1, 2):
+        df, dc = ldims[lv - 1], ldims[lv]
+        use_mask = int(not fem.masks_consistent(df, dc, masks[lv - 1], masks[lv]))
+        ne = int(np.prod(dc))
+        m64 = torch.zeros(576 * ne, dtype=torch.float64, device=DEV)
+        m32 = torch.zeros(576 * ne, dtype=torch.float32, device=DEV)
+        L.call("tg_galerkin_elems", L.grid(df), L.grid(dc), L.ptr(t(masks[lv - 1])), use_mask,
+               L.ptr(t(E)) if lv == 1 else None, L.ptr(fine_m), L.ptr(m64), L.ptr(m32), stream())
+        fr, Kc, _ = levels[lv]
+        nd = 3 * int(np.prod([d + 1 for d in dc]))
+        u = np.zeros(nd)
+        u[fr] = rng.standard_normal(fr.size)
+        y = _apply_stored(dc, masks[lv], m32, u)
+        assert rel_l2(y[fr], Kc @ u[fr]) < 5e-6, lv
+        assert np.array_equal(fem.free_from_node_mask(masks[lv]), fr)
+        if lv == 1 and not point and all(d % 2 == 0 for d in dims):
+            yq = torch.zeros(nd, dtype=torch.float32, device=DEV)
+            op = L.TgOp32(L.TG_OP_Q8_F32, L.grid(dc), L.ptr(t(masks[1])), L.ptr(t(E.astype(np.float32))), None)
+            L.call("tg_apply32", op, L.ptr(t(u.astype(np.float32))), L.ptr(yq),
+                   L.ep(L.TG_EP_STORE), stream())
+            assert rel_l2(h(yq)[fr], Kc @ u[fr]) < 5e-6
+            d32 = torch.zeros(nd, dtype=torch.float32, device=DEV)
+            L.call("tg_diag_inv_q8", L.grid(dc), L.ptr(t(masks[1])), L.ptr(t(E)), L.ptr(d32), stream())
+            assert rel_inf(h(d32)[fr], 1.0 / Kc.diagonal()) < 1e-6
+        d32 = torch.zeros(nd, dtype=torch.float32, device=DEV)
+        L.call("tg_diag_inv_stored", L.grid(dc), L.ptr(t(masks[lv])), L.ptr(m64), L.ptr(d32), stream())
+        assert rel_inf(h(d32)[fr], 1.0 / Kc.diagonal()) < 1e-6
+        fine_m = m64
+        if lv == 2:
+            # dense coarsest: assembly, Cholesky inverse, application
+            fidx = -np.ones(nd, np.int32)
+            fidx[fr] = np.arange(fr.size, dtype=np.int32)
+            A = torch.zeros(fr.size * fr.size, dtype=torch.float64, device=DEV)
+            L.call("tg_assemble_dense", L.grid(dc), L.ptr(m64), L.ptr(t(fidx)), fr.size, L.ptr(A), stream())
+            Kd = Kc.toarray()
+            assert np.abs(h(A).reshape(Kd.shape) - Kd).max() < 1e-12 * np.abs(Kd).max()
+            L.call("tg_dense_spd_inverse", L.ptr(A), fr.size, stream())
+            Ainv = h(A).reshape(Kd.shape)
+            assert np.abs(Ainv @ Kd - np.eye(fr.size)).max() < 1e-8
+            rc = np.zeros(nd, np.float32)
+            rc[fr] = rng.standard_normal(fr.size)
+            ec = torch.full((nd,), np.nan, dtype=torch.float32, device=DEV)
+            L.call("tg_coarsest_apply", L.ptr(A), fr.size, L.ptr(t(fr.astype(np.int32))), nd, L.ptr(t(rc)),
+                   L.ptr(ec), stream())
+            e = h(ec)
+            assert rel_l2(e[fr], np.linalg.solve(Kd, rc[fr].astype(float))) < 1e-6
+            assert np.all(e[np.setdiff1d(np.arange(nd), fr)] == 0.0)
+
+
+def test_not_spd_is_reported():
+    A = t(np.array([1.0, 2.0, 2.0, 1.0]))
+    with pytest.raises(L.TopoHipError, match="Cholesky"):
+        L.call("tg_dense_spd_inverse", L.ptr(A), 2, stream())
+
+
+@pytest.mark.parametrize("rmin", [1.5, 2.0, 2.7])
+def test_filter_and_transpose(rmin):
+    dims = (9, 5, 7)
+    n = int(np.prod(dims))
+    H = oracle.build_density_filter(*dims, rmin)
+    offs, w = fem.filter_stencil(rmin)
+    # Hs = W 1 from the raw stencil
+    ones = t(np.ones(n))
+    hsd = torch.zeros(n, dtype=torch.float64, device=DEV)
+    L.call("tg_stencil64", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(ones), None, None,
+           L.ptr(hsd), stream())
+    hs_inv = 1.0 / h(hsd)
+    x = np.random.default_rng(1).uniform(0, 1, n)
+    out = torch.zeros(n, dtype=torch.float64, device=DEV)
+    L.call("tg_stencil64", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(t(x)), None,
+           L.ptr(t(hs_inv)), L.ptr(out), stream())
+    assert rel_inf(h(out), H @ x) < 1e-14
+    L.call("tg_stencil64", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(t(x)),
+           L.ptr(t(hs_inv)), None, L.ptr(out), stream())
+    assert rel_inf(h(out), H.T @ x) < 1e-14
+
+
+def test_density_chain_and_sensitivity():
+    dims = (6, 4, 5)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(9)
+    rf = rng.uniform(0.0, 1.0, n)
+    outs = [torch.zeros(n, dtype=torch.float64, device=DEV) for _ in range(3)]
+    e32 = torch.zeros(n, dtype=torch.float32, device=DEV)
+    L.call("tg_density_chain", n, L.ptr(t(rf)), 1.0, 0.5, 3.0, 1e-9, 1.0, *[L.ptr(o) for o in outs], L.ptr(e32),
+           stream())
+    rp, dh, E = (h(o) for o in outs)
+    assert rel_inf(rp, oracle.heaviside(rf, 1.0)) < 1e-14
+    assert rel_inf(dh, oracle.heaviside_deriv(rf, 1.0)) < 1e-14
+    assert rel_inf(E, 1e-9 + (1 - 1e-9) * oracle.heaviside(rf, 1.0) ** 3) < 1e-14
+    ndof = 3 * int(np.prod([d + 1 for d in dims]))
+    U = rng.standard_normal(ndof)
+    ce = torch.zeros(n, dtype=torch.float64, device=DEV)
+    dcf = torch.zeros(n, dtype=torch.float64, device=DEV)
+    L.call("tg_sensitivity64", L.grid(dims), L.ptr(t(U)), L.ptr(t(rp)), L.ptr(t(dh)), 3.0, 1 - 1e-9, L.ptr(ce),
+           L.ptr(dcf), stream())
+    ce_ref = oracle.element_compliance(U, oracle.edof_table_3d(*dims))
+    assert rel_inf(h(ce), ce_ref) < 1e-13
+    assert rel_inf(h(dcf), -3.0 * (1 - 1e-9) * rp ** 2 * ce_ref * dh) < 1e-13
+
+
+def test_oc_bisection_matches_oracle():
+    dims = (10, 6, 5)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(12)
+    rho = rng.uniform(0.05, 0.95, n)
+    dc = -rng.uniform(0.01, 5.0, n)
+    dv = rng.uniform(0.5, 1.5, n)
+    H = oracle.build_density_filter(*dims, 1.5)
+    ref = oracle.oc_update_filtered(rho, dc, dv, 0.45, 0.2, H, 1.0, 0.5, True)
+    offs, w = fem.filter_stencil(1.5)
+    # Hs = W 1 from the raw stencil on the host
+    Wsum = np.zeros(dims)
+    for (dx, dy, dz), wk in zip(offs, w):
+        c = np.zeros(dims)
+        c[max(0, -dx):dims[0] - max(0, dx), max(0, -dy):dims[1] - max(0, dy), max(0, -dz):dims[2] - max(0, dz)] = 1
+        Wsum += wk * c
+    hs_inv = 1.0 / Wsum.ravel()
+    q, lo, hi, rn = (torch.zeros(n, dtype=torch.float64, device=DEV) for _ in range(4))
+    state = torch.zeros(8, dtype=torch.float64, device=DEV)
+    chg = torch.zeros(1, dtype=torch.float64, device=DEV)
+    drho = t(rho)
+    L.call("tg_oc_prepare", n, L.ptr(drho), L.ptr(t(dc)), L.ptr(t(dv)), 0.2, L.ptr(q), L.ptr(lo), L.ptr(hi),
+           L.ptr(state), stream())
+    dhs = t(hs_inv)
+    for _ in range(100):
+        L.call("tg_oc_bisect_step", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(dhs),
+               L.ptr(q), L.ptr(lo), L.ptr(hi), 1.0, 0.5, 0.45, 1, L.ptr(state), stream())
+    L.call("tg_oc_finalize", n, L.ptr(drho), L.ptr(q), L.ptr(lo), L.ptr(hi), L.ptr(state), L.ptr(rn), L.ptr(chg),
+           stream())
+    got = h(rn)
+    assert np.abs(got - ref).max() < 1e-9
+    assert h(chg)[0] == pytest.approx(np.abs(ref - rho).max(), abs=1e-9)
+    assert 55 <= h(state)[4] <= 100
+
+
+def test_krylov_vector_kernels():
+    n = 100_003
+    rng = np.random.default_rng(2)
+    a, b, c = (rng.standard_normal(n).astype(np.float32) for _ in range(3))
+    s = torch.zeros(8, dtype=torch.float64, device=DEV)
+    L.call("tg_dot32", n, L.ptr(t(a)), L.ptr(t(b)), L.ptr(t(c)), L.ptr(s[0:1]), L.ptr(s[1:2]), stream())
+    r = h(s)
+    assert r[0] == pytest.approx(float(a.astype(float) @ b), rel=1e-12)
+    assert r[1] == pytest.approx(float(a.astype(float) @ c), rel=1e-12)
+    # determinism: same launch, same bits
+    L.call("tg_dot32", n, L.ptr(t(a)), L.ptr(t(b)), None, L.ptr(s[2:3]), None, stream())
+    assert h(s)[2] == r[0]
+    x, rr_, p, q = (t(v) for v in (a, b, c, a[::-1].copy()))
+    s[0], s[1] = 2.0, 4.0
+    L.call("tg_pcg_xr", n, L.ptr(x), L.ptr(rr_), L.ptr(p), L.ptr(q), L.ptr(s[0:1]), L.ptr(s[1:2]), L.ptr(s[2:3]),
+           stream())
+    assert np.allclose(h(x), a + 0.5 * c, rtol=1e-6)
+    rnew = b - 0.5 * a[::-1]
+    assert np.allclose(h(rr_), rnew, rtol=1e-6, atol=1e-6)
+    assert h(s)[2] == pytest.approx(float(rnew.astype(float) @ rnew), rel=1e-6)

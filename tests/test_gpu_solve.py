@@ -1,0 +1,75 @@
+"""State solve and full SIMP loop on the GPU against the FP64 oracle and the
+reference's golden run_simp trajectories."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import topogpu_amd as tg
+from topogpu_amd.hip.state import HipStateSolver
+
+from gpu_util import DEV, cantilever_bc, h, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
+@pytest.mark.parametrize("dims,levels", [((16, 8, 8), 3), ((24, 12, 6), 4), ((13, 7, 9), 3)])
+def test_state_solve_matches_direct(gpu, precision, dims, levels):
+    free, F, ndof = cantilever_bc(dims, point_support=True)
+    rng = np.random.default_rng(21)
+    E = 1e-9 + rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3
+    U_ref = oracle.solve_state(oracle.edof_table_3d(*dims), E, free, F, ndof)
+    s = HipStateSolver(dims, free, F, precision=precision, n_levels=levels, tol=1e-9, max_krylov=400)
+    E64 = torch.from_numpy(E).to(DEV)
+    s.set_modulus(E64, E64.float(), float(E.max()))
+    st = s.solve()
+    assert st.converged, st
+    assert st.rel_residual <= 1e-9
+    U = h(s.u)
+    assert rel_l2(U, U_ref) < 1e-6
+    assert s.compliance() == pytest.approx(float(F @ U_ref), rel=1e-8)
+    # warm start from the converged solution returns without a Krylov step
+    st2 = s.solve()
+    assert st2.iters == 0 and st2.converged
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+def test_simp_c0_matches_reference_golden(gpu, golden, precision):
+    """24x12x6 cantilever, 3 OC iterations: compliance history and physical
+    density against the reference CPU run_simp (spsolve) trajectory."""
+    g = golden("c0_cantilever_24x12x6")
+    prob = tg.unit_cantilever((24, 12, 6), volfrac=0.30, rmin=1.5)
+    res = tg.SIMPSolver(backend="hip", tol=1e-8, max_krylov=800, max_iter=3, move=0.2,
+                        precision=precision).solve(prob)
+    comp = np.array([r["compliance"] for r in res.history])
+    assert np.allclose(comp, g["compliance_history"], rtol=1e-6, atol=0), comp / g["compliance_history"] - 1
+    assert np.abs(res.rho_final - g["rho_final"]).max() < 1e-5
+    assert res.summary["final_compliance"] == pytest.approx(float(g["final_compliance"]), rel=1e-6)
+    assert res.summary["n_iter"] == int(g["n_iter"])
+
+
+def test_simp_c0_at_stated_tolerance(gpu, golden):
+    """Same case at the configured PCG tolerance 1e-5 (relative residual)."""
+    g = golden("c0_cantilever_24x12x6")
+    prob = tg.unit_cantilever((24, 12, 6), volfrac=0.30, rmin=1.5)
+    res = tg.SIMPSolver(backend="hip", tol=1e-5, max_iter=3, move=0.2).solve(prob)
+    comp = np.array([r["compliance"] for r in res.history])
+    assert np.allclose(comp, g["compliance_history"], rtol=1e-5, atol=0), comp / g["compliance_history"] - 1
+    assert np.abs(res.rho_final - g["rho_final"]).max() < 1e-3
+
+
+def test_simp_odd_block_matches_reference_golden(gpu, golden):
+    """10x5x3 block, side load, rmin 2, 6 iterations (odd coarse levels)."""
+    g = golden("c0b_block_10x5x3_rmin2")
+    spec = tg.ProblemSpec(Lx=10.0, Ly=5.0, Lz=3.0, nelx=10, nely=5, nelz=3, volfrac=0.4,
+                          supports=[tg.EdgeSupport("left", "fixed")],
+                          loads=[tg.PointLoad(x=10.0, y=0.0, z=1.0, fy=-1.0)])
+    prob = tg.TopologyProblem("block", spec, filter_radius=2.0)
+    res = tg.SIMPSolver(backend="hip", tol=1e-8, max_iter=6, move=0.2, n_levels=2).solve(prob)
+    comp = np.array([r["compliance"] for r in res.history])
+    assert np.allclose(comp, g["compliance_history"], rtol=1e-6, atol=0)
+    assert np.abs(res.rho_final - g["rho_final"]).max() < 1e-5

@@ -1,0 +1,243 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is chosen so that a plausible restatement mistake fails it:
+closed-form entries and energies for KE, brute force for the filter, golden
+trajectories from the reference's own run_simp, finite differences for the
+sensitivity chain, exact reproduction of linear fields for the prolongation.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from oracle import gmg, simp
+from oracle.bc import generate_bc
+
+NU = 0.3
+LAM = NU / ((1 + NU) * (1 - 2 * NU))
+MU = 1 / (2 * (1 + NU))
+
+
+def _nodes_xyz():
+    return 2.0 * simp.HEX_CORNERS.astype(float) - 1.0
+
+
+def test_ke_closed_form_entries():
+    K = oracle.KE_UNIT_3D
+    # int (dN0/dx)^2 = 2/9 on [-1,1]^3 and int dN0/dx dN0/dy = 1/6
+    assert K[0, 0] == pytest.approx((LAM + 4 * MU) * 2 / 9, rel=1e-14)
+    assert K[0, 1] == pytest.approx((LAM + MU) / 6, rel=1e-14)
+    assert np.allclose(K, K.T, atol=1e-15)
+
+
+def test_ke_rigid_modes_and_rank():
+    K = oracle.KE_UNIT_3D
+    X = _nodes_xyz()
+    modes = []
+    for c in range(3):
+        t = np.zeros((8, 3))
+        t[:, c] = 1.0
+        modes.append(t.ravel())
+    for axis in range(3):
+        w = np.zeros(3)
+        w[axis] = 1.0
+        modes.append(np.cross(w[None, :], X).ravel())
+    for m in modes:
+        assert np.abs(K @ m).max() < 1e-13
+    assert np.linalg.matrix_rank(K, tol=1e-10) == 18
+    assert np.linalg.eigvalsh(K).min() > -1e-13
+
+
+def test_ke_homogeneous_strain_energy():
+    """u = eps x gives u^T K u = vol * sigma:eps with vol = 8."""
+    K = oracle.KE_UNIT_3D
+    X = _nodes_xyz()
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        A = rng.standard_normal((3, 3))
+        eps = 0.5 * (A + A.T)
+        u = (X @ eps.T).ravel()
+        sig = LAM * np.trace(eps) * np.eye(3) + 2 * MU * eps
+        assert u @ K @ u == pytest.approx(8.0 * np.sum(sig * eps), rel=1e-12)
+
+
+def test_ke_quadrature_exact():
+    assert np.abs(simp.ke_unit_3d(n_gauss=3) - oracle.KE_UNIT_3D).max() < 1e-14
+
+
+def test_edof_closed_form():
+    edof = oracle.edof_table_3d(3, 2, 4)
+    e = (2 * 2 + 1) * 4 + 3          # element (2,1,3)
+    node0 = (2 * 3 + 1) * 5 + 3
+    assert list(edof[e, :3]) == [3 * node0, 3 * node0 + 1, 3 * node0 + 2]
+    node6 = (3 * 3 + 2) * 5 + 4      # corner (1,1,1)
+    assert list(edof[e, 18:21]) == [3 * node6, 3 * node6 + 1, 3 * node6 + 2]
+
+
+@pytest.mark.parametrize("dims,rmin", [((4, 3, 2), 1.5), ((5, 4, 3), 2.0), ((6, 2, 3), 2.6)])
+def test_filter_matches_dense_bruteforce(dims, rmin):
+    """H equals the dense max(0, r-d) row-normalised matrix of
+    src/topogpu/filter.py:10-27 (brute force over all pairs)."""
+    H = oracle.build_density_filter(*dims, rmin).toarray()
+    ix, iy, iz = np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
+    c = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1).astype(float)
+    d = np.linalg.norm(c[:, None, :] - c[None, :, :], axis=2)
+    W = np.maximum(0.0, rmin - d)
+    Hd = W / W.sum(1, keepdims=True)
+    assert np.abs(H - Hd).max() < 1e-15
+    assert np.allclose(H.sum(1), 1.0, atol=1e-14)
+
+
+def test_filter_19_point_interior_weights():
+    offs, w = simp.filter_offsets(1.5)
+    assert len(w) == 19
+    rs = sorted(np.round(w, 12))
+    assert rs.count(0.5) == 6 and rs.count(1.5) == 1
+    assert rs.count(round(1.5 - np.sqrt(2.0), 12)) == 12
+    H = oracle.build_density_filter(5, 5, 5, 1.5)
+    centre = (2 * 5 + 2) * 5 + 2
+    assert H[centre, centre] == pytest.approx(1.5 / (1.5 + 3.0 + 12 * (1.5 - np.sqrt(2))), rel=1e-14)
+
+
+def test_heaviside_closed_form():
+    x = np.array([0.0, 0.5, 1.0])
+    for beta in (1.0, 8.0):
+        assert np.allclose(oracle.heaviside(x, beta), [0.0, 0.5, 1.0], atol=1e-15)
+    x = np.linspace(0.05, 0.95, 7)
+    h = 1e-6
+    fd = (oracle.heaviside(x + h, 4.0) - oracle.heaviside(x - h, 4.0)) / (2 * h)
+    assert np.allclose(fd, oracle.heaviside_deriv(x, 4.0), rtol=1e-8)
+    assert np.array_equal(oracle.heaviside(x, 0.0), x)
+
+
+def _tiny_case(dims=(4, 3, 2)):
+    nelx, nely, nelz = dims
+    from types import SimpleNamespace as NS
+    spec = NS(nelx=nelx, nely=nely, nelz=nelz, Lx=float(nelx), Ly=float(nely), Lz=float(nelz),
+              supports=[NS(edge="left", constraint="fixed")],
+              loads=[NS(x=float(nelx), y=0.0, z=float(nelz), fx=0.3, fy=-1.0, fz=0.2)])
+    _, free, F, ndof = generate_bc(spec)
+    return free, F, ndof
+
+
+def _compliance_of_design(rho, H, beta, edof, free, F, ndof, penal=3.0):
+    rf = H @ rho
+    rp = oracle.heaviside(rf, beta)
+    E = oracle.EMIN + (1 - oracle.EMIN) * rp ** penal
+    U = oracle.solve_state(edof, E, free, F, ndof)
+    return float(F @ U)
+
+
+def test_sensitivity_chain_matches_central_fd():
+    dims = (4, 3, 2)
+    free, F, ndof = _tiny_case(dims)
+    edof = oracle.edof_table_3d(*dims)
+    H = oracle.build_density_filter(*dims, 1.5)
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.2, 0.9, np.prod(dims))
+    beta = 2.0
+    _, _, _, _, ex = oracle.fea_and_sensitivity(rho, 3.0, 0.2, beta, 0.5, True, H, volfrac=0.5, edof=edof,
+                                                free=free, F=F, ndof=ndof)
+    h = 1e-6
+    for e in rng.choice(rho.size, 6, replace=False):
+        rp, rm = rho.copy(), rho.copy()
+        rp[e] += h
+        rm[e] -= h
+        fd = (_compliance_of_design(rp, H, beta, edof, free, F, ndof)
+              - _compliance_of_design(rm, H, beta, edof, free, F, ndof)) / (2 * h)
+        assert ex["dc_design"][e] == pytest.approx(fd, rel=2e-6)
+
+
+def test_oc_bounds_volume_and_kkt():
+    dims = (6, 4, 3)
+    n = int(np.prod(dims))
+    H = oracle.build_density_filter(*dims, 1.5)
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(0.05, 0.95, n)
+    dc = -rng.uniform(0.1, 5.0, n)
+    dv = rng.uniform(0.5, 1.5, n)
+    move, vf = 0.2, 0.4
+    rn = oracle.oc_update_filtered(rho, dc, dv, vf, move, H, 1.0, 0.5, True)
+    lo = np.maximum(rho - move, 1e-3)
+    hi = np.minimum(rho + move, 1.0)
+    assert np.all(rn >= lo - 1e-15) and np.all(rn <= hi + 1e-15)
+    vol = oracle.heaviside(H @ rn, 1.0).mean()
+    assert abs(vol - vf) < 1e-6
+    inner = (rn > lo + 1e-12) & (rn < hi - 1e-12)
+    ratio = rn[inner] / (rho[inner] * np.sqrt(-dc[inner] / dv[inner]))
+    assert ratio.std() / ratio.mean() < 1e-12      # one common multiplier
+
+
+def test_oc_mean_volume_variant():
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(0.1, 0.9, 200)
+    dc = -rng.uniform(0.1, 3.0, 200)
+    target = rho.mean() - 0.03          # reachable within the move limit
+    rn = oracle.oc_update_mean(rho, dc, target, 0.1, 1e-3)
+    assert abs(rn.mean() - target) < 1e-6
+    assert np.all(rn <= np.minimum(rho + 0.1, 1) + 1e-15)
+
+
+def test_matfree_matches_assembled():
+    dims = (5, 3, 4)
+    free, F, ndof = _tiny_case(dims)
+    edof = oracle.edof_table_3d(*dims)
+    rng = np.random.default_rng(7)
+    E = rng.uniform(1e-3, 1.0, np.prod(dims))
+    K = oracle.assemble_k(edof, E, ndof)
+    Kff = K[free][:, free]
+    mask = np.zeros(ndof, bool)
+    mask[free] = True
+    u = rng.standard_normal(ndof)
+    y = simp.matfree_matvec(u, E, edof, mask)
+    assert np.abs(y[free] - Kff @ u[free]).max() < 1e-13 * np.abs(y).max()
+    assert np.all(y[~mask] == 0)
+    d = simp.matfree_diag(E, edof, mask, ndof)
+    assert np.allclose(d[free], Kff.diagonal(), rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2"])
+def test_run_simp_matches_reference_golden(golden, name):
+    g = golden(name)
+    cfg = {"c0_cantilever_24x12x6": ((24, 12, 6), 0.30, 1.5, 3),
+           "c0b_block_10x5x3_rmin2": ((10, 5, 3), 0.40, 2.0, 6)}[name]
+    (dims, vf, rmin, iters) = cfg
+    r = oracle.run_simp(*dims, vf, g["free"], g["F"], rmin=rmin, move=0.2, max_iter=iters, min_iter=2,
+                        tol=1e-3)
+    assert np.allclose(r["compliance_history"], g["compliance_history"], rtol=1e-9, atol=0)
+    assert r["n_iter"] == int(g["n_iter"])
+    assert np.abs(r["rho_final"] - g["rho_final"]).max() < 1e-9
+    assert r["final_compliance"] == pytest.approx(float(g["final_compliance"]), rel=1e-9)
+
+
+@pytest.mark.parametrize("nf,nc", [(8, 4), (7, 3), (3, 1), (1, 1), (2, 1)])
+def test_prolongation_axis_reproduces_linear(nf, nc):
+    W = gmg._axis_weights(nf, nc)
+    rs = W.sum(1)
+    W = W / rs[:, None]
+    coarse_x = np.arange(nc + 1) * 2.0
+    fine = W @ coarse_x
+    # even dims: exact linear interpolation; odd tails copy the last parent
+    for i in range(nf + 1):
+        if i // 2 + (i & 1) <= nc:
+            assert fine[i] == pytest.approx(float(i))
+        else:
+            assert fine[i] == pytest.approx(coarse_x[-1])
+
+
+def test_galerkin_hierarchy_small():
+    dims = (8, 4, 4)
+    free, F, ndof = _tiny_case(dims)
+    edof = oracle.edof_table_3d(*dims)
+    E = np.random.default_rng(2).uniform(0.1, 1.0, np.prod(dims))
+    Kff = oracle.assemble_k(edof, E, ndof)[free][:, free]
+    levels = gmg.galerkin_coarse_operators(Kff, gmg.level_dims(*dims, 3), free)
+    for fr, Kc, P in levels[1:]:
+        assert abs(Kc - Kc.T).max() < 1e-12 * abs(Kc).max()
+        assert np.all(np.asarray(P.sum(1)).ravel() <= 1.0 + 1e-14)
+    # the x=0 face stays fixed on every level
+    fr1 = levels[1][0]
+    assert 0 not in fr1 and 1 not in fr1 and 2 not in fr1

@@ -1,0 +1,323 @@
+// Library state, constant tables and Krylov vector kernels.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "common.h"
+#include "mfma_layout.h"
+#include "reduce.h"
+
+namespace tg {
+
+__constant__ double c_ke64[576];
+__constant__ float c_ke32[576];
+__constant__ float c_q8_32[8 * 576];
+// BF16 bit patterns of the unit stiffness in MFMA A-operand order, see
+// mfma_layout.h: [hi/lo][M tile][lane][8].
+__constant__ uint16_t c_ke_bf16[2 * 2 * 64 * 8];
+
+namespace {
+thread_local char g_err[512] = "";
+constexpr int kMaxDevices = 64;
+bool g_ready[kMaxDevices] = {};
+Workspace g_ws[kMaxDevices] = {};
+
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) return -1;
+  return d;
+}
+
+uint16_t bf16_bits_rne(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);   // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float bf16_to_float(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+}  // namespace
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+bool ke_ready() {
+  const int d = current_device();
+  return d >= 0 && d < kMaxDevices && g_ready[d];
+}
+
+Workspace workspace() {
+  const int d = current_device();
+  return (d >= 0 && d < kMaxDevices) ? g_ws[d] : Workspace{nullptr, nullptr};
+}
+
+int check_grid(const tg_grid* g) {
+  TG_REQUIRE(g != nullptr, "grid is NULL");
+  TG_REQUIRE(g->nelx >= 1 && g->nely >= 1 && g->nelz >= 1, "grid dims must be >= 1 (got %d %d %d)",
+             g->nelx, g->nely, g->nelz);
+  const double nodes = (double)(g->nelx + 1) * (g->nely + 1) * (g->nelz + 1);
+  TG_REQUIRE(3.0 * nodes < 2147483647.0 * 64.0, "grid too large");
+  return TG_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+
+namespace {
+
+constexpr int kVecBlock = 256;
+
+int vec_blocks(long long n) {
+  long long b = (n + kVecBlock * 4 - 1) / (kVecBlock * 4);
+  if (b < 1) b = 1;
+  if (b > 4096) b = 4096;
+  return (int)b;
+}
+
+__global__ void k_fill32(long long n, float* x, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_fill64(long long n, double* x, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a, const float* b,
+                                                    const float* c, double* out, double* out2,
+                                                    Workspace ws) {
+  double v[2] = {0.0, 0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double ai = a[i];
+    v[0] += ai * (double)b[i];
+    if (c) v[1] += ai * (double)c[i];
+  }
+  double* const outs[2] = {out, out2};
+  grid_sum<2>(v, outs, ws);
+}
+
+__global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* a, const double* b,
+                                                    double* out, Workspace ws) {
+  double v[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    v[0] = fma(a[i], b[i], v[0]);
+  double* const outs[1] = {out};
+  grid_sum<1>(v, outs, ws);
+}
+
+__global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, float* r, const float* p,
+                                                     const float* q, const double* rz,
+                                                     const double* pq, double* rr, Workspace ws) {
+  const double den = pq[0];
+  const float alpha = (float)(den != 0.0 ? rz[0] / den : 0.0);
+  double v[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] = fmaf(alpha, p[i], x[i]);
+    const float ri = fmaf(-alpha, q[i], r[i]);
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  double* const outs[1] = {rr};
+  grid_sum<1>(v, outs, ws);
+}
+
+__global__ void k_pcg_dir(long long n, float* p, const float* z, const double* zq, const double* pq) {
+  const double den = pq[0];
+  const float beta = (float)(den != 0.0 ? -zq[0] / den : 0.0);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = fmaf(beta, p[i], z[i]);
+}
+
+__global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, float a, float* d, float* x, float* r) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float v = a * dinv[i] * rhs[i];
+    d[i] = v;
+    x[i] = v;
+    if (r) r[i] = rhs[i];
+  }
+}
+
+__global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v, const float* kv, const float* dinv,
+                                                         double* vdv, Workspace ws) {
+  double acc[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double vi = v[i];
+    acc[0] += vi * vi / (double)dinv[i];
+    v[i] = dinv[i] * kv[i];
+  }
+  double* const outs[1] = {vdv};
+  grid_sum<1>(acc, outs, ws);
+}
+
+__global__ void k_axpy_f32_to_f64(long long n, double* x, const float* d) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] += (double)d[i];
+}
+
+}  // namespace
+}  // namespace tg
+
+using namespace tg;
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" const char* tg_last_error(void) { return g_err; }
+extern "C" int tg_version(void) { return 1; }
+
+extern "C" int tg_init(const double* ke) {
+  TG_REQUIRE(ke != nullptr, "ke_unit_host is NULL");
+  const int dev = current_device();
+  TG_REQUIRE(dev >= 0 && dev < kMaxDevices, "no current HIP device");
+  float ke32[576];
+  for (int i = 0; i < 576; ++i) ke32[i] = (float)ke[i];
+
+  // Galerkin child matrices Q_s = P_s^T KE P_s, child s = (sx,sy,sz) at
+  // index 4*sx + 2*sy + sz. Fine local position t = s + corner offset in
+  // {0,1,2} takes weights (1,0), (1/2,1/2), (0,1) from coarse nodes (0,1).
+  static const double w1d[3][2] = {{1.0, 0.0}, {0.5, 0.5}, {0.0, 1.0}};
+  float q8[8 * 576];
+  for (int s = 0; s < 8; ++s) {
+    const int sx = s >> 2 & 1, sy = s >> 1 & 1, sz = s & 1;
+    double P[8][8];
+    for (int c = 0; c < 8; ++c)
+      for (int C = 0; C < 8; ++C)
+        P[c][C] = w1d[sx + corner_x(c)][corner_x(C)] * w1d[sy + corner_y(c)][corner_y(C)] *
+                  w1d[sz + corner_z(c)][corner_z(C)];
+    for (int A = 0; A < 24; ++A)
+      for (int B = 0; B < 24; ++B) {
+        double acc = 0.0;
+        for (int a = 0; a < 8; ++a)
+          for (int b = 0; b < 8; ++b) {
+            const double pa = P[a][A / 3], pb = P[b][B / 3];
+            if (pa == 0.0 || pb == 0.0) continue;
+            acc += pa * ke[(3 * a + A % 3) * 24 + 3 * b + B % 3] * pb;
+          }
+        q8[s * 576 + A * 24 + B] = (float)acc;
+      }
+  }
+
+  // KE in MFMA A-operand order with hi/lo BF16 split.
+  uint16_t kb[2 * 2 * 64 * 8];
+  for (int t = 0; t < 2; ++t)
+    for (int l = 0; l < 64; ++l)
+      for (int v = 0; v < 8; ++v) {
+        const int row = mfma_a_row(t, l), col = mfma_a_col(l, v);
+        const int nr = row >> 2, cr = row & 3, nc = col >> 2, cc = col & 3;
+        const float x = (cr < 3 && cc < 3) ? ke32[(3 * nr + cr) * 24 + 3 * nc + cc] : 0.0f;
+        const uint16_t hi = bf16_bits_rne(x);
+        const uint16_t lo = bf16_bits_rne(x - bf16_to_float(hi));
+        kb[((0 * 2 + t) * 64 + l) * 8 + v] = hi;
+        kb[((1 * 2 + t) * 64 + l) * 8 + v] = lo;
+      }
+
+  TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke64), ke, sizeof(double) * 576));
+  TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke32), ke32, sizeof(ke32)));
+  TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_q8_32), q8, sizeof(q8)));
+  TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke_bf16), kb, sizeof(kb)));
+  if (!g_ws[dev].partials) {
+    void* p = nullptr;
+    TG_HIP_CHECK(hipMalloc(&p, sizeof(double) * kMaxPartials + 256));
+    g_ws[dev].partials = (double*)p;
+    g_ws[dev].ticket = (unsigned*)((char*)p + sizeof(double) * kMaxPartials);
+    TG_HIP_CHECK(hipMemset(g_ws[dev].ticket, 0, 256));
+  }
+  TG_HIP_CHECK(hipDeviceSynchronize());
+  g_ready[dev] = true;
+  return TG_OK;
+}
+
+#define TG_STREAM(s) ((hipStream_t)(s))
+#define TG_NEED_INIT() TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device")
+
+extern "C" int tg_fill32(int64_t n, float* x, float v, void* s) {
+  TG_REQUIRE(n >= 0 && (n == 0 || x), "bad fill32 args");
+  if (n == 0) return TG_OK;
+  hipLaunchKernelGGL(k_fill32, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, v);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_fill64(int64_t n, double* x, double v, void* s) {
+  TG_REQUIRE(n >= 0 && (n == 0 || x), "bad fill64 args");
+  if (n == 0) return TG_OK;
+  hipLaunchKernelGGL(k_fill64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, v);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_dot32(int64_t n, const float* a, const float* b, const float* c, double* out,
+                        double* out2, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && a && b && out && (c == nullptr) == (out2 == nullptr), "bad dot32 args");
+  hipLaunchKernelGGL(k_dot32, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b,
+                     c, out, out2, workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_dot64(int64_t n, const double* a, const double* b, double* out, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && a && b && out, "bad dot64 args");
+  hipLaunchKernelGGL(k_dot64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b,
+                     out, workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_pcg_xr(int64_t n, float* x, float* r, const float* p, const float* q,
+                         const double* rz, const double* pq, double* rr, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && x && r && p && q && rz && pq && rr, "bad pcg_xr args");
+  hipLaunchKernelGGL(k_pcg_xr, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r,
+                     p, q, rz, pq, rr, workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const double* pq,
+                          void* s) {
+  TG_REQUIRE(n > 0 && p && z && zq && pq, "bad pcg_dir args");
+  hipLaunchKernelGGL(k_pcg_dir, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, p, z,
+                     zq, pq);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_axpy_f32_to_f64(int64_t n, double* x, const float* d, void* s) {
+  TG_REQUIRE(n > 0 && x && d, "bad axpy args");
+  hipLaunchKernelGGL(k_axpy_f32_to_f64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s),
+                     (long long)n, x, d);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, float* d, float* x, float* r,
+                              void* s) {
+  TG_REQUIRE(n > 0 && rhs && dinv && d && x && r != rhs, "bad cheb_init args");
+  hipLaunchKernelGGL(k_cheb_init, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, rhs, dinv, a,
+                     d, x, r);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && v && kv && dinv && vdv, "bad power_step args");
+  hipLaunchKernelGGL(k_power_step, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
+                     vdv, workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}

@@ -1,0 +1,335 @@
+// Geometric multigrid on the structured hex grid: trilinear transfers,
+// Galerkin coarse element matrices and the dense coarsest solve.
+//
+// Coarse grid per axis: n_c = max(1, n_f / 2). Prolongation per axis (fine
+// node i, coarse node I): even i takes weight 1 from I = i/2; odd i takes 1/2
+// from i/2 and i/2 + 1; parents past n_c are dropped and each fine row is
+// rescaled to sum 1 (the reference's row-normalised trilinear P,
+// solver_v2.py:62-149). P is the Kronecker product of the three axes, so the
+// normalisation factorises per axis.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "common.h"
+#include "reduce.h"
+
+namespace tg {
+namespace {
+
+struct AxisW {
+  int lo, hi;      // coarse parents (hi == lo when single)
+  double wlo, whi;
+};
+
+__host__ __device__ __forceinline__ AxisW axis_weights(int i, int nc) {
+  AxisW a;
+  a.lo = i >> 1;
+  if ((i & 1) == 0) {
+    a.hi = a.lo; a.wlo = 1.0; a.whi = 0.0;
+  } else if (a.lo + 1 <= nc) {
+    a.hi = a.lo + 1; a.wlo = 0.5; a.whi = 0.5;
+  } else {
+    a.hi = a.lo; a.wlo = 1.0; a.whi = 0.0;   // lone parent after normalisation
+  }
+  return a;
+}
+
+// Weight of coarse node I for fine node i along one axis (0 if not a parent).
+__host__ __device__ __forceinline__ double axis_w(int i, int I, int nc) {
+  const AxisW a = axis_weights(i, nc);
+  double w = 0.0;
+  if (a.lo == I) w += a.wlo;
+  if (a.hi == I && a.hi != a.lo) w += a.whi;
+  return w;
+}
+
+__global__ void k_prolong(Grid f, Grid c, const uint8_t* __restrict__ mask_f, const float* __restrict__ ec,
+                          float* __restrict__ xf, int accumulate) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= f.nodes()) return;
+  const int k = (int)(n % f.NZ()), j = (int)((n / f.NZ()) % f.NY()), i = (int)(n / ((long long)f.NZ() * f.NY()));
+  const AxisW ax = axis_weights(i, c.nx), ay = axis_weights(j, c.ny), az = axis_weights(k, c.nz);
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int I = (a & 4) ? ax.hi : ax.lo, J = (a & 2) ? ay.hi : ay.lo, K = (a & 1) ? az.hi : az.lo;
+    const double w = ((a & 4) ? ax.whi : ax.wlo) * ((a & 2) ? ay.whi : ay.wlo) * ((a & 1) ? az.whi : az.wlo);
+    if (w == 0.0) continue;
+    const float* p = ec + 3 * c.node(I, J, K);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = fmaf((float)w, p[q], acc[q]);
+  }
+  const unsigned m = mask_f[n];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float base = accumulate ? xf[3 * n + q] : 0.0f;
+    xf[3 * n + q] = (m >> q & 1u) ? base + acc[q] : base;
+  }
+}
+
+__global__ void k_restrict(Grid f, Grid c, const uint8_t* __restrict__ mask_c, const float* __restrict__ rf,
+                           float* __restrict__ rc) {
+  const long long N = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (N >= c.nodes()) return;
+  const int K = (int)(N % c.NZ()), J = (int)((N / c.NZ()) % c.NY()), I = (int)(N / ((long long)c.NZ() * c.NY()));
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int di = -1; di <= 1; ++di) {
+    const int i = 2 * I + di;
+    if (i < 0 || i > f.nx) continue;
+    const double wx = axis_w(i, I, c.nx);
+    if (wx == 0.0) continue;
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j > f.ny) continue;
+      const double wy = axis_w(j, J, c.ny);
+      if (wy == 0.0) continue;
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int k = 2 * K + dk;
+        if (k < 0 || k > f.nz) continue;
+        const double wz = axis_w(k, K, c.nz);
+        if (wz == 0.0) continue;
+        const float w = (float)(wx * wy * wz);
+        const float* p = rf + 3 * f.node(i, j, k);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, p[q], acc[q]);
+      }
+    }
+  }
+  const unsigned m = mask_c[N];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) rc[3 * N + q] = (m >> q & 1u) ? acc[q] : 0.f;
+}
+
+// One block per coarse element: K_C = sum_f (M_f P_f)^T K_f (M_f P_f).
+__global__ void __launch_bounds__(256) k_galerkin(Grid f, Grid c, const uint8_t* __restrict__ mask_f, int use_mask,
+                                                 const double* __restrict__ fE, const double* __restrict__ fmats,
+                                                 double* __restrict__ out64, float* __restrict__ out32) {
+  __shared__ double sK[576];
+  __shared__ double sT[576];
+  __shared__ double sP[8][8];   // local P: fine corner x coarse corner
+  __shared__ double sM[24];
+  const long long C = blockIdx.x;
+  const int CK = (int)(C % c.nz), CJ = (int)((C / c.nz) % c.ny), CI = (int)(C / ((long long)c.nz * c.ny));
+  // children ranges per axis
+  const int fx0 = 2 * CI, fx1 = (CI == c.nx - 1) ? f.nx - 1 : 2 * CI + 1;
+  const int fy0 = 2 * CJ, fy1 = (CJ == c.ny - 1) ? f.ny - 1 : 2 * CJ + 1;
+  const int fz0 = 2 * CK, fz1 = (CK == c.nz - 1) ? f.nz - 1 : 2 * CK + 1;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int fi = fx0; fi <= fx1; ++fi)
+    for (int fj = fy0; fj <= fy1; ++fj)
+      for (int fk = fz0; fk <= fz1; ++fk) {
+        const long long fe = f.elem(fi, fj, fk);
+        for (int t = threadIdx.x; t < 576; t += blockDim.x)
+          sK[t] = fmats ? fmats[576 * fe + t] : fE[fe] * c_ke64[t];
+        if (threadIdx.x < 64) {
+          const int a = threadIdx.x >> 3, b = threadIdx.x & 7;   // fine corner a, coarse corner b
+          sP[a][b] = axis_w(fi + corner_x(a), CI + corner_x(b), c.nx) *
+                     axis_w(fj + corner_y(a), CJ + corner_y(b), c.ny) *
+                     axis_w(fk + corner_z(a), CK + corner_z(b), c.nz);
+        }
+        if (threadIdx.x < 24) {
+          const int a = threadIdx.x / 3, q = threadIdx.x % 3;
+          double mv = 1.0;
+          if (use_mask) {
+            const long long n = f.node(fi + corner_x(a), fj + corner_y(a), fk + corner_z(a));
+            mv = (mask_f[n] >> q & 1u) ? 1.0 : 0.0;
+          }
+          sM[threadIdx.x] = mv;
+        }
+        __syncthreads();
+        // T[a][B] = sum_b K[a][b] (MP)[b][B]
+        for (int t = threadIdx.x; t < 576; t += blockDim.x) {
+          const int a = t / 24, B = t % 24, q = B % 3;
+          double s = 0.0;
+#pragma unroll
+          for (int nb = 0; nb < 8; ++nb) s += sK[a * 24 + 3 * nb + q] * sM[3 * nb + q] * sP[nb][B / 3];
+          sT[t] = s;
+        }
+        __syncthreads();
+        // K_C[A][B] += sum_a (MP)[a][A] T[a][B]
+        for (int r = 0; r < 3; ++r) {
+          const int t = threadIdx.x + r * 256;
+          if (t < 576) {
+            const int A = t / 24, B = t % 24, q = A % 3;
+            double s = 0.0;
+#pragma unroll
+            for (int na = 0; na < 8; ++na) s += sP[na][A / 3] * sM[3 * na + q] * sT[(3 * na + q) * 24 + B];
+            acc[r] += s;
+          }
+        }
+        __syncthreads();
+      }
+  for (int r = 0; r < 3; ++r) {
+    const int t = threadIdx.x + r * 256;
+    if (t < 576) {
+      out64[576 * C + t] = acc[r];
+      if (out32) out32[576 * C + t] = (float)acc[r];
+    }
+  }
+}
+
+// Row-wise dense assembly: thread per DOF of the grid.
+__global__ void k_assemble_dense(Grid g, const double* __restrict__ mats, const int* __restrict__ fidx, int nf,
+                                 double* __restrict__ A) {
+  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (d >= 3 * g.nodes()) return;
+  const int row = fidx[d];
+  if (row < 0) return;
+  const long long n = d / 3;
+  const int r = (int)(d % 3);
+  const int k = (int)(n % g.NZ()), j = (int)((n / g.NZ()) % g.NY()), i = (int)(n / ((long long)g.NZ() * g.NY()));
+  double* Ar = A + (long long)row * nf;
+  for (int o = 0; o < 8; ++o) {
+    const int ei = i - corner_x(o), ej = j - corner_y(o), ek = k - corner_z(o);
+    if (ei < 0 || ei >= g.nx || ej < 0 || ej >= g.ny || ek < 0 || ek >= g.nz) continue;
+    const double* K = mats + 576 * g.elem(ei, ej, ek) + (3 * o + r) * 24;
+    for (int cc = 0; cc < 8; ++cc) {
+      const long long m = g.node(ei + corner_x(cc), ej + corner_y(cc), ek + corner_z(cc));
+      for (int q = 0; q < 3; ++q) {
+        const int col = fidx[3 * m + q];
+        if (col >= 0) Ar[col] += K[3 * cc + q];
+      }
+    }
+  }
+}
+
+// A (n x n, upper triangle of the row-major array valid) -> full symmetric.
+__global__ void k_symmetrize_from_upper(double* A, int n) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int r = (int)(t / n), c = (int)(t % n);
+  if (r > c) A[t] = A[(long long)c * n + r];
+}
+
+// One wavefront per row: ec[free[i]] = sum_j Ainv[i][j] rc[free[j]].
+__global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const int* __restrict__ fl,
+                                 const float* __restrict__ rc, float* __restrict__ ec) {
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* a = Ainv + (long long)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s = fma(a[j], (double)rc[fl[j]], s);
+  s = wave_sum(s);
+  if (lane == 0) ec[fl[row]] = (float)s;
+}
+
+struct SolverHandles {
+  rocblas_handle h = nullptr;
+};
+SolverHandles g_handles[64];
+
+}  // namespace
+}  // namespace tg
+
+using namespace tg;
+
+#define TG_NEED_INIT() TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device")
+
+static int check_pair(const tg_grid* fine, const tg_grid* coarse) {
+  if (int rc = check_grid(fine)) return rc;
+  if (int rc = check_grid(coarse)) return rc;
+  TG_REQUIRE(coarse->nelx == (fine->nelx / 2 > 1 ? fine->nelx / 2 : 1) &&
+                 coarse->nely == (fine->nely / 2 > 1 ? fine->nely / 2 : 1) &&
+                 coarse->nelz == (fine->nelz / 2 > 1 ? fine->nelz / 2 : 1),
+             "coarse grid must be max(1, n/2) of the fine grid");
+  return TG_OK;
+}
+
+extern "C" int tg_restrict32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c, const float* rf,
+                             float* rc, void* s) {
+  if (int r = check_pair(fine, coarse)) return r;
+  TG_REQUIRE(mask_c && rf && rc, "restrict: NULL argument");
+  const Grid f = to_grid(fine), c = to_grid(coarse);
+  hipLaunchKernelGGL(k_restrict, dim3((unsigned)((c.nodes() + 255) / 256)), dim3(256), 0, (hipStream_t)s, f, c,
+                     mask_c, rf, rc);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_prolong32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, const float* ec,
+                            float* xf, int accumulate, void* s) {
+  if (int r = check_pair(fine, coarse)) return r;
+  TG_REQUIRE(mask_f && ec && xf, "prolong: NULL argument");
+  const Grid f = to_grid(fine), c = to_grid(coarse);
+  hipLaunchKernelGGL(k_prolong, dim3((unsigned)((f.nodes() + 255) / 256)), dim3(256), 0, (hipStream_t)s, f, c,
+                     mask_f, ec, xf, accumulate);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_galerkin_elems(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, int use_mask,
+                                 const double* fine_E, const double* fine_mats, double* out64, float* out32,
+                                 void* s) {
+  TG_NEED_INIT();
+  if (int r = check_pair(fine, coarse)) return r;
+  TG_REQUIRE(out64 && (fine_E || fine_mats) && (!use_mask || mask_f), "galerkin: NULL argument");
+  const Grid f = to_grid(fine), c = to_grid(coarse);
+  TG_REQUIRE(c.elems() < (1LL << 31), "too many coarse elements");
+  hipLaunchKernelGGL(k_galerkin, dim3((unsigned)c.elems()), dim3(256), 0, (hipStream_t)s, f, c, mask_f, use_mask,
+                     fine_E, fine_mats, out64, out32);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const int32_t* fidx, int32_t nf, double* A,
+                                 void* s) {
+  if (int r = check_grid(gp)) return r;
+  TG_REQUIRE(mats && fidx && A && nf > 0, "assemble_dense: bad argument");
+  const Grid g = to_grid(gp);
+  if (int r = tg_fill64((int64_t)nf * nf, A, 0.0, s)) return r;
+  const long long nd = 3 * g.nodes();
+  hipLaunchKernelGGL(k_assemble_dense, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, mats,
+                     fidx, nf, A);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
+  TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
+  int dev = 0;
+  TG_HIP_CHECK(hipGetDevice(&dev));
+  TG_REQUIRE(dev >= 0 && dev < 64, "device id out of range");
+  if (!g_handles[dev].h) {
+    if (rocblas_create_handle(&g_handles[dev].h) != rocblas_status_success)
+      return set_error(TG_ERR_HIP, "rocblas_create_handle failed");
+  }
+  rocblas_handle h = g_handles[dev].h;
+  if (rocblas_set_stream(h, (hipStream_t)s) != rocblas_status_success)
+    return set_error(TG_ERR_HIP, "rocblas_set_stream failed");
+  rocblas_int* dinfo = nullptr;
+  TG_HIP_CHECK(hipMallocAsync((void**)&dinfo, sizeof(rocblas_int), (hipStream_t)s));
+  // Row-major symmetric A is its own column-major transpose; work on the
+  // column-major lower triangle, i.e. the row-major upper triangle.
+  rocblas_int info = 0;
+  rocblas_status st = rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, dinfo);
+  if (st == rocblas_status_success) {
+    TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
+    TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+  }
+  if (st == rocblas_status_success && info == 0) {
+    st = rocsolver_dpotri(h, rocblas_fill_lower, n, A, n, dinfo);
+    if (st == rocblas_status_success) {
+      TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
+      TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+    }
+  }
+  TG_HIP_CHECK(hipFreeAsync(dinfo, (hipStream_t)s));
+  if (st != rocblas_status_success) return set_error(TG_ERR_HIP, "rocsolver potrf/potri status %d", (int)st);
+  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense Cholesky failed (info=%d)", (int)info);
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)s, A,
+                     n);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_coarsest_apply(const double* Ainv, int32_t n, const int32_t* fl, int64_t n_full, const float* rc,
+                                 float* ec, void* s) {
+  TG_REQUIRE(Ainv && fl && rc && ec && n > 0 && n_full >= n, "coarsest_apply: bad argument");
+  if (int r = tg_fill32(n_full, ec, 0.0f, s)) return r;
+  hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)s, Ainv, n, fl, rc,
+                     ec);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}

@@ -1,0 +1,266 @@
+"""Galerkin geometric multigrid on the device (V-cycle preconditioner).
+
+Restates the hierarchy of ``GalerkinMatFreeGMG`` (src/gpu_fem/multigrid_v4.py:
+446-1042) on gfx950 kernels:
+
+* level 0 is the matrix-free hex8 operator (BF16 MFMA or FP32);
+* level 1 is the exact Galerkin operator P^T K P; when the fine dims are even
+  and the Dirichlet masks are aligned with the coarse grid its element
+  matrices are sum_s E_child(s) * Q_s and are applied implicitly from the
+  fine E (no stored matrices on the V-cycle path), otherwise stored;
+* levels >= 2 are stored Galerkin element matrices built on the device from
+  the level above (replacing the SpGEMM R @ (K @ P), multigrid_v4.py:747-752);
+* the coarsest level is solved with an FP64 dense Cholesky factorisation
+  turned into an explicit inverse once per setup (multigrid_v4.py:774-811).
+
+Smoothing is Chebyshev-Jacobi on [lambda_max * lower_frac, lambda_max] with
+the coefficient recurrence of ``_chebyshev_steps`` (multigrid_v4.py:860-889),
+written in residual-update form: the residual is carried along
+(r <- r - A d) and each operator application acts on the smoother step d,
+never on the full iterate. A low-precision product then errs relative to
+the step rather than to the whole (smooth, large) iterate, and the
+pre-smoother hands its residual to the restriction without another product.
+Each step is one operator launch with the update fused into its epilogue.
+The one smooth product of the cycle, A * (P e_c) after the coarse
+correction, uses the accurate operator of the level. Coarse levels run in
+FP32.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import fem
+from . import lib as L
+
+
+class _Level:
+    pass
+
+
+KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
+                L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
+                L.TG_OP_STORED_F32: "stored_f32"}
+
+_EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
+                   L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8}
+
+
+def hex_bytes(n_dof: int, n_elem: int, mode: int) -> float:
+    """Algorithmic HBM bytes of one level-0 operator launch: FP32 input, E,
+    the node mask, plus the epilogue's own vector reads and writes (4 bytes
+    per DOF per vector)."""
+    return 4.0 * n_dof + 4.0 * n_elem + n_dof / 3.0 + _EP_VEC_TRAFFIC[mode] * n_dof
+
+
+def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: int) -> None:
+    """tg_apply32, timed with events when a timer is attached (level 0 only)."""
+    if timer is None or lv != 0:
+        L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream)
+        return
+    timer.launch(KERNEL_NAMES[op.kind], hex_bytes(n_dof, n_elem, ep.mode),
+                 lambda: L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream))
+
+
+class GMG:
+    def __init__(self, dims, mask0: np.ndarray, *, n_levels: int = 4, fine_op: int = L.TG_OP_HEX_BF16,
+                 fine_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
+                 cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
+                 power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7):
+        if n_levels < 2:
+            raise ValueError("the GMG preconditioner needs n_levels >= 2")
+        self.dev = device
+        self.fine_degree = max(1, int(fine_degree))
+        self.coarse_degree = max(1, int(coarse_degree))
+        self.frac = float(cheb_lower_frac)
+        self.safety = float(cheb_safety)
+        self.power_iters = power_iters
+        self.coarse_power_iters = coarse_power_iters
+        self.dims = fem.level_dims(dims, n_levels)
+        self.levels: list[_Level] = []
+        rng = np.random.default_rng(seed)
+        mask = mask0
+        for lv, d in enumerate(self.dims):
+            if lv > 0:
+                mask = fem.injection_mask(self.dims[lv - 1], d, self.levels[-1].mask_host)
+            Lv = _Level()
+            Lv.dims = d
+            Lv.grid = L.grid(d)
+            Lv.mask_host = mask
+            Lv.mask = torch.from_numpy(mask).to(device)
+            Lv.nodes = int(np.prod([n + 1 for n in d]))
+            Lv.n = 3 * Lv.nodes
+            Lv.n_elem = int(np.prod(d))
+            f32 = dict(dtype=torch.float32, device=device)
+            Lv.x = torch.zeros(Lv.n, **f32)       # iterate
+            Lv.r = torch.zeros(Lv.n, **f32)       # residual carried by the smoother
+            Lv.da = torch.zeros(Lv.n, **f32)      # smoother steps (ping-pong)
+            Lv.db = torch.zeros(Lv.n, **f32)
+            Lv.c = torch.zeros(Lv.n, **f32)       # prolonged coarse correction
+            Lv.rhs = torch.zeros(Lv.n, **f32)
+            Lv.dinv = torch.ones(Lv.n, **f32)
+            Lv.v0 = torch.from_numpy(rng.standard_normal(Lv.n).astype(np.float32)).to(device)
+            Lv.lam = float("nan")
+            Lv.mats64 = None
+            Lv.mats32 = None
+            self.levels.append(Lv)
+        self.levels[0].kind = fine_op
+        self.levels[0].kind_accurate = fine_accurate_op if fine_accurate_op is not None else fine_op
+        for lv in range(1, n_levels):
+            f, c = self.levels[lv - 1], self.levels[lv]
+            c.consistent = fem.masks_consistent(f.dims, c.dims, f.mask_host, c.mask_host)
+            c.kind = L.TG_OP_STORED_F32
+        l1 = self.levels[1]
+        if n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
+            l1.kind = L.TG_OP_Q8_F32
+        for lv in range(1, n_levels):
+            self.levels[lv].kind_accurate = self.levels[lv].kind
+        last = self.levels[-1]
+        free = fem.free_from_node_mask(last.mask_host)
+        self.n_coarse = int(free.size)
+        if self.n_coarse == 0:
+            raise ValueError("coarsest level has no free DOFs")
+        fidx = -np.ones(last.n, dtype=np.int32)
+        fidx[free] = np.arange(free.size, dtype=np.int32)
+        self.coarse_fidx = torch.from_numpy(fidx).to(device)
+        self.coarse_list = torch.from_numpy(free.astype(np.int32)).to(device)
+        self.coarse_A = torch.zeros(self.n_coarse * self.n_coarse, dtype=torch.float64, device=device)
+        self.scal = torch.zeros(8, dtype=torch.float64, device=device)
+        self._emax_at_estimate = float("nan")
+        self.is_setup = False
+        self.timer = None          # optional KernelTimer (level-0 launches)
+        self.E64 = self.E32 = None
+
+    # ------------------------------------------------------------------ ops
+    def op(self, lv: int, accurate: bool = False) -> L.TgOp32:
+        Lv = self.levels[lv]
+        kind = Lv.kind_accurate if accurate else Lv.kind
+        if kind == L.TG_OP_STORED_F32:
+            return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), None, L.ptr(Lv.mats32))
+        return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), L.ptr(self.E32), None)
+
+    def _launch(self, lv, op, u, y, ep, stream):
+        Lv = self.levels[lv]
+        launch_op(self.timer, lv, op, L.ptr(u), L.ptr(y), ep, stream, Lv.n, Lv.n_elem)
+
+    # ---------------------------------------------------------------- setup
+    def setup(self, E64: torch.Tensor, E32: torch.Tensor, emax: float, stream) -> None:
+        """Per-SIMP-iteration setup (GalerkinMatFreeGMG.setup, multigrid_v4.py:657-813)."""
+        self.E64, self.E32 = E64, E32
+        lv0 = self.levels[0]
+        L.call("tg_diag_inv_hex", lv0.grid, L.ptr(lv0.mask), L.ptr(E64), None, L.ptr(lv0.dinv), stream)
+        prev = self._emax_at_estimate
+        reestimate = np.isnan(prev) or abs(emax - prev) / max(abs(prev), 1e-12) > 0.10
+        last_lv = len(self.levels) - 1
+        for lv in range(1, len(self.levels)):
+            f, c = self.levels[lv - 1], self.levels[lv]
+            if c.mats64 is None:
+                c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
+            need32 = c.kind == L.TG_OP_STORED_F32 and lv < last_lv
+            if need32 and c.mats32 is None:
+                c.mats32 = torch.empty(576 * c.n_elem, dtype=torch.float32, device=self.dev)
+            L.call("tg_galerkin_elems", f.grid, c.grid, L.ptr(f.mask), int(not c.consistent),
+                   L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64), L.ptr(c.mats64),
+                   L.ptr(c.mats32) if need32 else None, stream)
+            if lv < last_lv:
+                if c.kind == L.TG_OP_Q8_F32:
+                    L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
+                else:
+                    L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
+        last = self.levels[-1]
+        L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
+               L.ptr(self.coarse_A), stream)
+        L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
+        if reestimate:
+            for lv in range(last_lv):
+                iters = self.power_iters if lv == 0 else self.coarse_power_iters
+                self.levels[lv].lam = self.estimate_lambda(lv, iters, stream)
+            self._emax_at_estimate = emax
+        self.is_setup = True
+
+    def estimate_lambda(self, lv: int, iters: int, stream) -> float:
+        """lambda_max(D^-1 A) by power iteration in the D inner product
+        (_estimate_fine_lambda_max, multigrid_v4.py:273-295). The start
+        vector is a fixed random vector rather than all ones: on a uniform
+        density the all-ones vector is almost a rigid translation, has almost
+        no high-frequency content and 20 steps then underestimate lambda_max
+        badly on large grids."""
+        Lv = self.levels[lv]
+        v, kv = Lv.x, Lv.r
+        v.copy_(Lv.v0)
+        lam = 1.0
+        s = self.scal
+        ep = L.ep(L.TG_EP_DOT, dot=s[0:1])
+        for _ in range(iters):
+            self._launch(lv, self.op(lv, accurate=True), v, kv, ep, stream)
+            L.call("tg_power_step32", Lv.n, L.ptr(v), L.ptr(kv), L.ptr(Lv.dinv), L.ptr(s[1:2]), stream)
+            vkv, vdv = s[0:2].tolist()
+            if vdv <= 1e-300:
+                break
+            lam = vkv / vdv
+        return max(lam, 1e-6)
+
+    # --------------------------------------------------------------- cycle
+    def _cheb_coeffs(self, lv: int, degree: int):
+        """Per-step (a1, a2) of the recurrence alpha_0 = 1/sigma,
+        alpha_k = 1/(sigma - delta^2 alpha_{k-1}/4), d_k = alpha_k z + alpha_k
+        delta^2 alpha_{k-1}/4 d_{k-1} (multigrid_v4.py:870-887)."""
+        Lv = self.levels[lv]
+        lmax = (Lv.lam if np.isfinite(Lv.lam) else 1.0) * self.safety
+        lmin = self.frac * lmax
+        sigma, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+        alpha = 1.0 / sigma
+        out = [(alpha, 0.0)]
+        for _ in range(1, degree):
+            a_new = 1.0 / (sigma - delta * delta * alpha * 0.25)
+            out.append((a_new, a_new * delta * delta * alpha * 0.25))
+            alpha = a_new
+        return out
+
+    def _presmooth(self, lv: int, rhs, degree: int, stream):
+        """x from zero; leaves r = rhs - A x in Lv.r. Returns the last step buffer."""
+        Lv = self.levels[lv]
+        co = self._cheb_coeffs(lv, degree)
+        L.call("tg_cheb_init32", Lv.n, L.ptr(rhs), L.ptr(Lv.dinv), co[0][0], L.ptr(Lv.da), L.ptr(Lv.x), L.ptr(Lv.r),
+               stream)
+        d, dn = Lv.da, Lv.db
+        op = self.op(lv)
+        for a1, a2 in co[1:]:
+            self._launch(lv, op, d, None, L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
+            d, dn = dn, d
+        self._launch(lv, op, d, None, L.ep(L.TG_EP_RESID_UPDATE, r=Lv.r), stream)
+
+    def _postsmooth(self, lv: int, degree: int, stream):
+        """x += c (the prolonged correction, in Lv.c) then Chebyshev from there."""
+        Lv = self.levels[lv]
+        co = self._cheb_coeffs(lv, degree)
+        # r -= A c ; d1 = alpha0 Dinv r ; x += c + d1   (accurate operator: c is smooth)
+        self._launch(lv, self.op(lv, accurate=True), Lv.c, None,
+                     L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r, a1=co[0][0], a2=0.0, a3=1.0), stream)
+        d, dn = Lv.da, Lv.db
+        op = self.op(lv)
+        for a1, a2 in co[1:]:
+            self._launch(lv, op, d, None, L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
+            d, dn = dn, d
+
+    def _vcycle(self, lv: int, rhs, stream):
+        Lv = self.levels[lv]
+        if lv == len(self.levels) - 1:
+            L.call("tg_coarsest_apply", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
+                   L.ptr(rhs), L.ptr(Lv.x), stream)
+            return Lv.x
+        deg = self.fine_degree if lv == 0 else self.coarse_degree
+        self._presmooth(lv, rhs, deg, stream)
+        C = self.levels[lv + 1]
+        L.call("tg_restrict32", Lv.grid, C.grid, L.ptr(C.mask), L.ptr(Lv.r), L.ptr(C.rhs), stream)
+        ec = self._vcycle(lv + 1, C.rhs, stream)
+        L.call("tg_prolong32", Lv.grid, C.grid, L.ptr(Lv.mask), L.ptr(ec), L.ptr(Lv.c), 0, stream)
+        self._postsmooth(lv, deg, stream)
+        return Lv.x
+
+    def apply(self, r, stream):
+        """z = M r; returns a level-0 buffer valid until the next apply."""
+        if not self.is_setup:
+            raise RuntimeError("GMG.setup() must be called before apply()")
+        return self._vcycle(0, r, stream)

@@ -1,0 +1,151 @@
+"""One SIMP design step on the device: filter -> projection -> modulus ->
+state solve -> compliance sensitivity -> filter transpose -> OC bisection.
+
+Follows ``_fea_and_sensitivity`` and ``_oc_update`` of the reference CPU path
+(src/gpu_fem/pub_simp_solver.py:319-377). Everything except the state solve's
+convergence test and a few scalars stays on the GPU.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import fem
+from . import lib as L
+from .state import HipStateSolver
+
+E0, EMIN = 1.0, 1.0e-9
+
+
+@dataclass
+class StepResult:
+    compliance: float
+    change: float
+    iters: int
+    restarts: int
+    rel_residual: float
+    converged: bool
+    oc_steps: int
+
+
+class HipSIMP:
+    def __init__(self, dims, free, F, *, volfrac: float, rmin: float = 1.5, penal: float = 3.0,
+                 move: float = 0.2, use_heaviside: bool = True, beta: float = 1.0, eta: float = 0.5,
+                 rho0: np.ndarray | None = None, **solver_kw):
+        self.dims = tuple(int(d) for d in dims)
+        self.n_elem = int(np.prod(self.dims))
+        self.volfrac, self.rmin, self.penal, self.move = float(volfrac), float(rmin), float(penal), float(move)
+        self.use_heaviside, self.beta, self.eta = bool(use_heaviside), float(beta), float(eta)
+        self.fea = HipStateSolver(self.dims, free, F, **solver_kw)
+        self.dev = self.fea.dev
+        self.grid = self.fea.grid
+        offs, w = fem.filter_stencil(self.rmin)
+        self.offs = np.ascontiguousarray(offs.ravel(), dtype=np.int32)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        self.m = int(w.size)
+        self.hs_inv = torch.from_numpy(1.0 / self._row_sums()).to(self.dev)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        rho = np.full(self.n_elem, self.volfrac) if rho0 is None else np.asarray(rho0, dtype=np.float64)
+        self.rho = torch.from_numpy(rho.copy()).to(self.dev)
+        self.rho_new = torch.zeros(self.n_elem, **f64)
+        self.rho_f = torch.zeros(self.n_elem, **f64)
+        self.rho_phys = torch.zeros(self.n_elem, **f64)
+        self.dh = torch.zeros(self.n_elem, **f64)
+        self.E64 = torch.zeros(self.n_elem, **f64)
+        self.E32 = torch.zeros(self.n_elem, dtype=torch.float32, device=self.dev)
+        self.dcf = torch.zeros(self.n_elem, **f64)
+        self.dc = torch.zeros(self.n_elem, **f64)
+        self.dv = torch.zeros(self.n_elem, **f64)
+        self.q = torch.zeros(self.n_elem, **f64)
+        self.lo = torch.zeros(self.n_elem, **f64)
+        self.hi = torch.zeros(self.n_elem, **f64)
+        self.oc_state = torch.zeros(8, **f64)
+        self.scal = torch.zeros(4, **f64)
+
+    def _row_sums(self) -> np.ndarray:
+        """Hs = W 1 on the host (geometry only)."""
+        nx, ny, nz = self.dims
+        hs = np.zeros((nx, ny, nz))
+        for (dx, dy, dz), wk in zip(self.offs.reshape(-1, 3), self.w):
+            cnt = np.zeros((nx, ny, nz))
+            sx = slice(max(0, -dx), nx - max(0, dx))
+            sy = slice(max(0, -dy), ny - max(0, dy))
+            sz = slice(max(0, -dz), nz - max(0, dz))
+            cnt[sx, sy, sz] = 1.0
+            hs += wk * cnt
+        return hs.ravel()
+
+    @property
+    def stream(self):
+        return self.fea.stream
+
+    # ------------------------------------------------------------------
+    def filter(self, x, out, transpose: bool = False) -> None:
+        """out = H x (or H^T x)."""
+        ins = self.hs_inv if transpose else None
+        outs = None if transpose else self.hs_inv
+        L.call("tg_stencil64", self.grid, self.m, self.offs.ctypes.data, self.w.ctypes.data, L.ptr(x),
+               L.ptr(ins), L.ptr(outs), L.ptr(out), self.stream)
+
+    def physical(self, rho=None) -> None:
+        """rho_f = H rho; rho_phys, dh, E (pub_simp_solver.py:347-350)."""
+        self.filter(self.rho if rho is None else rho, self.rho_f)
+        beta = self.beta if self.use_heaviside else 0.0
+        L.call("tg_density_chain", self.n_elem, L.ptr(self.rho_f), beta, self.eta, self.penal, EMIN, E0,
+               L.ptr(self.rho_phys), L.ptr(self.dh), L.ptr(self.E64), L.ptr(self.E32), self.stream)
+
+    def state(self):
+        """Set moduli, build the hierarchy and solve; returns (compliance, stats)."""
+        emax = float(self.E64.max().item())
+        self.fea.set_modulus(self.E64, self.E32, emax)
+        stats = self.fea.solve()
+        return self.fea.compliance(), stats
+
+    def sensitivities(self) -> None:
+        """dc = H^T(dc_phys * dh), dv = H^T dh (pub_simp_solver.py:361-371)."""
+        L.call("tg_sensitivity64", self.grid, L.ptr(self.fea.u), L.ptr(self.rho_phys), L.ptr(self.dh), self.penal,
+               E0 - EMIN, None, L.ptr(self.dcf), self.stream)
+        self.filter(self.dcf, self.dc, transpose=True)
+        self.filter(self.dh, self.dv, transpose=True)
+
+    def oc(self, max_steps: int = 100) -> tuple[float, int]:
+        """Filtered-volume OC bisection (pub_simp_solver.py:319-340)."""
+        st = self.stream
+        L.call("tg_oc_prepare", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), L.ptr(self.dv), self.move,
+               L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), L.ptr(self.oc_state), st)
+        use_h = int(self.use_heaviside and self.beta > 1e-6)
+        done = 0
+        steps = 0
+        while steps < max_steps:
+            chunk = min(64 if steps == 0 else 8, max_steps - steps)
+            for _ in range(chunk):
+                L.call("tg_oc_bisect_step", self.grid, self.m, self.offs.ctypes.data, self.w.ctypes.data,
+                       L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
+                       self.volfrac, use_h, L.ptr(self.oc_state), st)
+            steps += chunk
+            done = float(self.oc_state[3].item())
+            if done:
+                break
+        L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
+               L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
+        change = float(self.scal[0].item())
+        n_steps = int(self.oc_state[4].item())
+        self.rho, self.rho_new = self.rho_new, self.rho
+        return change, n_steps
+
+    def step(self) -> StepResult:
+        self.physical()
+        c, stats = self.state()
+        self.sensitivities()
+        change, n_oc = self.oc()
+        return StepResult(c, change, stats.iters, stats.restarts, stats.rel_residual, stats.converged, n_oc)
+
+    def evaluate(self):
+        """Final re-evaluation (pub_simp_solver.py:635-641): compliance and the
+        physical density of the current design."""
+        self.physical()
+        c, stats = self.state()
+        return c, self.rho_phys.cpu().numpy(), stats

@@ -1,0 +1,177 @@
+"""State solve K_ff u = F on the device: GMG-preconditioned flexible PCG in
+FP32 inside FP64 iterative refinement.
+
+Replaces ``SolverV4._solve_cupy_matfree`` (src/gpu_fem/solver_v4.py:147-269)
+with ``_cupy_pcg`` (src/gpu_fem/solver_v2.py:552-624) as the Krylov method:
+
+* outer loop (FP64): r = F - K u with the FP64 operator; stop when
+  ||r|| <= tol ||F|| (the reference's criterion, checked on the true FP64
+  residual instead of the recursive one);
+* inner loop (FP32 vectors, FP64 dot products): PCG on K d = r with the
+  level-0 operator in the chosen precision (see PRECISIONS) and the GMG
+  V-cycle as preconditioner. BF16 rounding inside the operators makes the
+  preconditioner slightly nonlinear, so the direction update uses the
+  flexible (Polak-Ribiere) beta = z_new.(r_new - r_old)/(z.r);
+* u += d, and the next restart recomputes the FP64 residual.
+
+All Krylov scalars stay on the device; the host reads one scalar per
+iteration for the convergence test. Vectors are full length with fixed DOFs
+held at zero.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import fem
+from . import lib as L
+from .gmg import GMG, launch_op
+
+# precision -> (Krylov operator, level-0 smoother operator)
+#   bf16       : split BF16 MFMA (hi/lo operands, 3 passes) for the PCG
+#                product, single-pass BF16 MFMA in the smoother
+#   bf16-plain : single-pass BF16 MFMA everywhere (the inner solve then
+#                stalls near the BF16 rounding level; kept for study)
+#   bf16x3     : split BF16 MFMA everywhere
+#   fp32       : FP32 vector FMA everywhere (the plain FP32 path)
+PRECISIONS = {
+    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16),
+    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16),
+    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3),
+    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32),
+}
+
+
+@dataclass
+class SolveStats:
+    iters: int            # preconditioner applications (the reference's count)
+    restarts: int         # FP64 residual evaluations
+    converged: bool
+    rel_residual: float   # ||F - K u|| / ||F|| in FP64
+
+
+class HipStateSolver:
+    """Device state solver for one structured grid and one load case."""
+
+    def __init__(self, dims, free: np.ndarray, F: np.ndarray, *, precision: str = "bf16", n_levels: int = 4,
+                 fine_degree: int = 2, coarse_degree: int = 2, cheb_lower_frac: float = 1.0 / 30.0,
+                 cheb_safety: float = 1.1,
+                 tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
+                 warm_start: bool = True, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}; got {precision!r}")
+        self.dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.dims = tuple(int(d) for d in dims)
+        self.ke = fem.unit_stiffness()
+        with torch.cuda.device(self.dev):
+            L.init_device(self.ke, self.dev.index)
+        self.grid = L.grid(self.dims)
+        self.n_nodes = int(np.prod([d + 1 for d in self.dims]))
+        self.n = 3 * self.n_nodes
+        self.n_elem = int(np.prod(self.dims))
+        mask = fem.node_mask_from_free(free, self.n_nodes)
+        self.mask = torch.from_numpy(mask).to(self.dev)
+        F = np.asarray(F, dtype=np.float64).copy()
+        F[np.setdiff1d(np.arange(self.n), free)] = 0.0
+        self.F = torch.from_numpy(F).to(self.dev)
+        self.bnorm = float(np.linalg.norm(F))
+        self.precision = precision
+        self.tol, self.max_krylov = float(tol), int(max_krylov)
+        self.inner_rtol, self.max_restarts = float(inner_rtol), int(max_restarts)
+        self.warm_start = warm_start
+        self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=PRECISIONS[precision][1],
+                       fine_accurate_op=PRECISIONS[precision][0], fine_degree=fine_degree,
+                       coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
+                       device=self.dev)
+        f32 = dict(dtype=torch.float32, device=self.dev)
+        self.u = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        self.r64 = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        self.r = torch.zeros(self.n, **f32)
+        self.d = torch.zeros(self.n, **f32)
+        self.p = torch.zeros(self.n, **f32)
+        self.q = torch.zeros(self.n, **f32)
+        # device scalars: 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 emax/compliance
+        self.s = torch.zeros(8, dtype=torch.float64, device=self.dev)
+        self.E64 = None
+        self.E32 = None
+        self.last: SolveStats | None = None
+
+    @property
+    def stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    # ------------------------------------------------------------------
+    def fine_op(self) -> L.TgOp32:
+        return L.TgOp32(PRECISIONS[self.precision][0], self.grid, L.ptr(self.mask), L.ptr(self.E32), None)
+
+    def set_modulus(self, E64: torch.Tensor, E32: torch.Tensor, emax: float) -> None:
+        """Element moduli for the next solve (FP64 for the true residual,
+        FP32 for the low-precision operators); builds the GMG hierarchy."""
+        self.E64, self.E32 = E64, E32
+        self.gmg.setup(E64, E32, emax, self.stream)
+
+    def residual64(self) -> float:
+        """r64 = F - K u (FP64), r = float(r64); returns ||r64||."""
+        L.call("tg_apply64", self.grid, L.ptr(self.mask), L.ptr(self.E64), L.ptr(self.u), L.ptr(self.F),
+               L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4:5]), self.stream)
+        return math.sqrt(max(float(self.s[4].item()), 0.0))
+
+    def solve(self) -> SolveStats:
+        st = self.stream
+        if not self.warm_start:
+            L.call("tg_fill64", self.n, L.ptr(self.u), 0.0, st)
+        target = self.tol * self.bnorm
+        its = restarts = 0
+        rnorm = self.residual64()
+        restarts += 1
+        if not np.isfinite(rnorm):
+            raise FloatingPointError("non-finite residual at the start of the state solve")
+        if rnorm > 10.0 * self.bnorm and self.warm_start:
+            # a warm start that is far off is worse than zero (solver_v4.py:201-209)
+            L.call("tg_fill64", self.n, L.ptr(self.u), 0.0, st)
+            rnorm = self.residual64()
+            restarts += 1
+        op = self.fine_op()
+        s = self.s
+        while rnorm > target and its < self.max_krylov and restarts <= self.max_restarts:
+            inner_target = max(self.inner_rtol * rnorm, 0.5 * target)
+            L.call("tg_fill32", self.n, L.ptr(self.d), 0.0, st)
+            z = self.gmg.apply(self.r, st)
+            its += 1
+            L.call("tg_dot32", self.n, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0:1]), None, st)
+            torch_copy(self.p, z)
+            while True:
+                ep = L.ep(L.TG_EP_DOT, dot=s[1:2])
+                launch_op(self.gmg.timer, 0, op, L.ptr(self.p), L.ptr(self.q), ep, st, self.n, self.n_elem)
+                L.call("tg_pcg_xr", self.n, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
+                       L.ptr(s[0:1]), L.ptr(s[1:2]), L.ptr(s[2:3]), st)
+                rr = float(s[2].item())
+                if not np.isfinite(rr):
+                    raise FloatingPointError("non-finite inner residual")
+                if math.sqrt(max(rr, 0.0)) <= inner_target or its >= self.max_krylov:
+                    break
+                z = self.gmg.apply(self.r, st)
+                its += 1
+                L.call("tg_dot32", self.n, L.ptr(z), L.ptr(self.r), L.ptr(self.q), L.ptr(s[0:1]), L.ptr(s[3:4]), st)
+                L.call("tg_pcg_dir", self.n, L.ptr(self.p), L.ptr(z), L.ptr(s[3:4]), L.ptr(s[1:2]), st)
+            L.call("tg_axpy_f32_to_f64", self.n, L.ptr(self.u), L.ptr(self.d), st)
+            rnorm = self.residual64()
+            restarts += 1
+        self.last = SolveStats(its, restarts, bool(rnorm <= target), rnorm / max(self.bnorm, 1e-300))
+        return self.last
+
+    def compliance(self) -> float:
+        """F . u in FP64 (solver_v4.py:244)."""
+        L.call("tg_dot64", self.n, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5:6]), self.stream)
+        return float(self.s[5].item())
+
+
+def torch_copy(dst: torch.Tensor, src: torch.Tensor) -> None:
+    """Device-to-device copy (plumbing)."""
+    dst.copy_(src)

@@ -1,0 +1,144 @@
+"""Public SIMP facade with the ``hip`` backend.
+
+Mirrors ``SIMPSolver`` / ``OptimizationResult`` / ``solve`` of the reference
+(src/topogpu/solver.py:39-253): same constructor signature, same validation
+errors, same result object. ``backend="hip"`` runs the whole SIMP iteration
+on an MI355X: the reference's CPU algorithm (``run_simp``, density filter +
+Heaviside projection + filtered-volume OC, src/gpu_fem/pub_simp_solver.py:
+412-657) with the state solve replaced by the GMG-PCG of ``topogpu_amd.hip``.
+BCs and loads come from the problem itself (the reference's ``run_simp``
+``bc_override`` path). Other backend names are not served here: the
+reference keeps serving them.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+from .bc import generate_bc
+from .evidence import EvidenceBundle
+from .problem import TopologyProblem
+
+HIP_BACKENDS = {"hip", "rocm"}
+
+
+@dataclass(slots=True)
+class SIMPModel:
+    penal: float = 3.0
+    rho_min: float = 1.0e-3
+
+
+@dataclass(slots=True)
+class OCOptimizer:
+    move: float = 0.10
+
+
+@dataclass(slots=True)
+class OptimizationResult:
+    problem: TopologyProblem
+    rho_final: np.ndarray
+    history: list[dict[str, Any]]
+    summary: dict[str, Any]
+
+    def save(self, path) -> EvidenceBundle:
+        """Evidence folder (src/topogpu/solver.py:46-61)."""
+        bundle = EvidenceBundle(Path(path))
+        bundle.write_history(self.history)
+        bundle.write_density(self.rho_final)
+        bundle.write_render_metadata({"density_threshold": 0.5, "scalar": "density",
+                                      "scalar_normalization": "per_case_0_1", "camera": "orthographic_default",
+                                      "source": "OptimizationResult.save"})
+        bundle.write_summary(self.summary)
+        bundle.write_manifest()
+        return bundle
+
+
+def grayness(rho: np.ndarray) -> float:
+    return float(4.0 * np.mean(rho * (1.0 - rho)))
+
+
+class SIMPSolver:
+    """Bounded SIMP loop; ``backend="hip"`` runs it on the GPU.
+
+    Extra keyword options of the hip backend (all optional):
+    precision ("bf16" | "bf16x3" | "fp32"), n_levels, fine_degree,
+    coarse_degree, inner_rtol, use_heaviside, beta, min_iter, conv_tol.
+    """
+
+    def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
+                 tol: float = 1.0e-5, max_krylov: int = 800, max_iter: int = 12, move: float = 0.10,
+                 rho_min: float = 1.0e-3, **hip_options: Any) -> None:
+        if optimizer != "oc":
+            raise ValueError("Only optimizer='oc' is implemented in v0.1.0.")
+        self.backend = backend
+        self.linear_solver = linear_solver
+        self.tol = tol
+        self.max_krylov = max_krylov
+        self.max_iter = max_iter
+        self.move = move
+        self.rho_min = rho_min
+        self.hip_options = dict(hip_options)
+
+    def solve(self, problem: TopologyProblem) -> OptimizationResult:
+        problem.validate()
+        if self.backend in HIP_BACKENDS:
+            return self._solve_hip(problem)
+        raise ValueError(f"Unknown backend {self.backend!r}.")
+
+    def _solve_hip(self, problem: TopologyProblem) -> OptimizationResult:
+        from .hip.simp import HipSIMP
+
+        if self.linear_solver not in {"auto", "pcg_gmg"}:
+            raise ValueError(f"backend='hip' supports linear_solver 'auto' or 'pcg_gmg'; got {self.linear_solver!r}")
+        spec = problem.spec
+        if not problem.is_3d:
+            raise ValueError("The hip backend currently requires a 3D problem.")
+        opts = dict(self.hip_options)
+        min_iter = int(opts.pop("min_iter", min(2, self.max_iter)))
+        conv_tol = float(opts.pop("conv_tol", 1.0e-3))
+        compliance_tol = float(opts.pop("compliance_tol", 5.0e-4))
+        window = int(opts.pop("compliance_window", 5))
+        bc = generate_bc(spec)
+        force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
+        rmin = problem.filter_radius or spec.rmin or 1.5
+        engine = HipSIMP(problem.nel, bc.free_dofs, force, volfrac=spec.volfrac, rmin=rmin, move=self.move,
+                         tol=self.tol, max_krylov=self.max_krylov, **opts)
+        history: list[dict[str, Any]] = []
+        comp_hist: list[float] = []
+        t_total = time.perf_counter()
+        iteration = 0
+        for iteration in range(1, self.max_iter + 1):
+            t0 = time.perf_counter()
+            step = engine.step()
+            wall = time.perf_counter() - t0
+            comp_hist.append(step.compliance)
+            rho_phys = engine.rho_phys.cpu().numpy()
+            history.append({
+                "iteration": iteration, "compliance": float(step.compliance), "outer_iters": int(step.iters),
+                "rho_mean": float(rho_phys.mean()), "rho_min": float(rho_phys.min()),
+                "rho_max": float(rho_phys.max()), "grayness": grayness(rho_phys), "wall_s": wall,
+                "outer_solver": "pcg", "change": float(step.change), "ir_restarts": int(step.restarts),
+                "rel_residual": float(step.rel_residual), "converged": bool(step.converged),
+            })
+            win = comp_hist[-window:]
+            conv_c = len(win) == window and abs(win[-1] - win[0]) / max(abs(win[0]), 1e-10) < compliance_tol
+            if step.change < conv_tol and conv_c and iteration >= min_iter:
+                break
+        final_c, rho_final, _ = engine.evaluate()
+        summary = {
+            "problem": problem.to_dict(), "backend": "hip",
+            "linear_solver": "pcg_gmg" if self.linear_solver == "auto" else self.linear_solver,
+            "precision": engine.fea.precision, "n_levels": len(engine.fea.gmg.levels), "n_iter": iteration,
+            "total_wall_s": time.perf_counter() - t_total, "final_compliance": float(final_c),
+            "final_grayness": grayness(rho_final), "final": history[-1] if history else {},
+        }
+        return OptimizationResult(problem, rho_final, history, summary)
+
+
+def solve(problem: TopologyProblem, **kwargs: Any) -> OptimizationResult:
+    return SIMPSolver(**kwargs).solve(problem)
