@@ -39,7 +39,7 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool):
+def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool, n_levels=None):
     import torch
 
     import topogpu_amd as tg
@@ -50,7 +50,8 @@ def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool):
     prob = tg.unit_cantilever(C2["dims"], volfrac=C2["volfrac"], rmin=C2["rmin"], load="line")
     bc = generate_bc(prob.spec)
     eng = HipSIMP(C2["dims"], bc.free_dofs, prob.force_override, volfrac=C2["volfrac"], rmin=C2["rmin"],
-                  move=C2["move"], precision=precision, n_levels=C2["n_levels"], fine_degree=C2["fine_degree"],
+                  move=C2["move"], precision=precision, n_levels=n_levels or C2["n_levels"],
+                  fine_degree=C2["fine_degree"],
                   coarse_degree=C2["coarse_degree"], tol=C2["tol"], max_krylov=C2["max_krylov"])
     for i in range(warmup):
         r = eng.step()
@@ -70,7 +71,8 @@ def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool):
         _log(f"{precision} step {i + 1}/{steps}: C={r.compliance:.6g} its={r.iters} restarts={r.restarts} "
              f"relres={r.rel_residual:.2e} oc={r.oc_steps}")
     stats = {
-        "iter_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+        "iter_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "n_levels": len(eng.fea.gmg.levels),
+        "coarsest_dofs": eng.fea.gmg.n_coarse,
         "pcg_iters_mean": float(np.mean([r.iters for r in rows])),
         "ir_restarts_mean": float(np.mean([r.restarts for r in rows])),
         "max_rel_residual": float(max(r.rel_residual for r in rows)),
@@ -247,6 +249,8 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-matvec", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the auto-depth hierarchy runs")
+    ap.add_argument("--levels", default=None, help="override the hierarchy depth of the headline run (int or auto)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode (see DESIGN.md for how it would "
@@ -258,12 +262,21 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
 
-    bf, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, timed_kernels=True)
+    levels = None if args.levels is None else (args.levels if args.levels == "auto" else int(args.levels))
+    bf, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, timed_kernels=True, n_levels=levels)
     _log(f"{args.precision}: {bf['iter_per_s']:.3f} iter/s")
     fp = None
     if not args.no_fp32:
         fp, _, _ = simp_bench("fp32", args.steps, args.warmup, timed_kernels=False)
         _log(f"fp32: {fp['iter_per_s']:.3f} iter/s")
+    variants = {}
+    if not args.no_variants:
+        # same problem and tolerance, hierarchy depth chosen so the dense
+        # FP64 coarsest level stays small (n_levels="auto")
+        for prec in (args.precision, "fp32"):
+            st, _, _ = simp_bench(prec, args.steps, args.warmup, timed_kernels=False, n_levels="auto")
+            variants[f"{prec}_auto_levels"] = st
+            _log(f"{prec} auto levels ({st['n_levels']}): {st['iter_per_s']:.3f} iter/s")
     mv = None if args.no_matvec else matvec_bench()
     # dominant level-0 kernel: most device time per SIMP step (launch count
     # from the in-loop timer x saturated per-launch duration)
@@ -288,7 +301,7 @@ def main() -> int:
         "solve": {k: bf[k] for k in ("pcg_iters_mean", "ir_restarts_mean", "max_rel_residual", "all_converged",
                                      "compliance_last")},
         "fp32_plain": fp, "speedup_vs_fp32": (bf["iter_per_s"] / fp["iter_per_s"]) if fp else None,
-        "matvec": mv, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "variants": variants, "matvec": mv, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

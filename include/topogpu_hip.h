@@ -159,6 +159,18 @@ int tg_galerkin_elems(const tg_grid* fine, const tg_grid* coarse, const uint8_t*
                       int use_mask, const double* fine_E, const double* fine_mats,
                       double* coarse_mats64, float* coarse_mats32, void* stream);
 
+/* Galerkin element matrices `levels_down` (1 or 2) levels below a fine hex
+ * grid when the Dirichlet masks are aligned with the coarse grids (no
+ * per-child masking needed) and the fine dims are divisible by
+ * 2^levels_down: K_C = sum_j E[desc_j(C)] * T_j with the 8 constant child
+ * matrices P_t^T KE P_t (one level) or the 64 grandchild matrices
+ * P_s^T P_t^T KE P_t P_s (two levels). One gather of the descendant moduli
+ * (work: n_coarse_elem * 8^levels_down doubles) and one DGEMM. Same result
+ * as tg_galerkin_elems applied once or twice with use_mask = 0; replaces
+ * assemble_level1_galerkin + SpGEMM (multigrid_v4.py:248-270, 747-752). */
+int tg_galerkin_from_fine(const tg_grid* fine, int levels_down, const double* fine_E, double* work,
+                          double* coarse_mats64, float* coarse_mats32, void* stream);
+
 /* Dense assembly of the free-DOF matrix of a level from element matrices:
  * A[i][j] (n_free x n_free, row-major FP64), free_index[dof] = compact index
  * or -1. Replaces K_coarsest.toarray() (multigrid_v4.py:781). */
@@ -236,9 +248,12 @@ int tg_sensitivity64(const tg_grid* g, const double* U, const double* rho_phys,
 /* OC bisection of pub_simp_solver.py:319-340 on the device.
  * oc_prepare: q = rho*sqrt(-min(dc,-1e-12)/max(dv,1e-12)),
  *   lo = max(rho-move, 1e-3), hi = min(rho+move, 1) ; state <- {l1=0, l2=1e9}.
- * oc_bisect_step: one bisection step (no-op once converged): lm=(l1+l2)/2,
- *   rn = clip(q/sqrt(lm), lo, hi), volume = sum(heaviside(H rn))/n, update
- *   the bracket, converged when l2-l1 < 1e-9. Filter as in tg_stencil64.
+ * oc_bisect_step: advances the bisection by up to 4 steps (no-op once
+ *   converged). Each step: lm=(l1+l2)/2, rn = clip(q/sqrt(lm), lo, hi),
+ *   volume = sum(heaviside(H rn))/n, update the bracket; converged when
+ *   l2-l1 < 1e-9 or after 100 steps. The 15 midpoints of the next 4 levels
+ *   are evaluated in one pass; the volume is monotone in lambda, so the
+ *   decisions equal those of 4 sequential steps. Filter as in tg_stencil64.
  * oc_finalize: rho_new = clip(q/sqrt(lm_last)); *change = max|rho_new-rho|.
  * state: 8 device doubles {l1, l2, lm, done, steps, vol, -, -}. */
 int tg_oc_prepare(int64_t n, const double* rho, const double* dc, const double* dv, double move,

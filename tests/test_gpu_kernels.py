@@ -286,6 +286,33 @@ def test_galerkin_levels_match_ptkp(dims, point):
             assert np.all(e[np.setdiff1d(np.arange(nd), fr)] == 0.0)
 
 
+@pytest.mark.parametrize("dims", [(8, 4, 4), (16, 8, 12)])
+def test_galerkin_from_fine_matches_two_step(dims):
+    """Gather + DGEMM path (8 child / 64 grandchild matrices) equals the
+    per-element Galerkin kernel applied once / twice without masks."""
+    free, F, ndof, E, mask0, rng = _setup(dims, 8)
+    d1, d2 = fem.coarse_dims(dims), fem.coarse_dims(fem.coarse_dims(dims))
+    n1, n2 = int(np.prod(d1)), int(np.prod(d2))
+    dE = t(E)
+    ref1 = torch.zeros(576 * n1, dtype=torch.float64, device=DEV)
+    ref2 = torch.zeros(576 * n2, dtype=torch.float64, device=DEV)
+    L.call("tg_galerkin_elems", L.grid(dims), L.grid(d1), L.ptr(t(mask0)), 0, L.ptr(dE), None, L.ptr(ref1), None,
+           stream())
+    m1 = fem.injection_mask(dims, d1, mask0)
+    L.call("tg_galerkin_elems", L.grid(d1), L.grid(d2), L.ptr(t(m1)), 0, None, L.ptr(ref1), L.ptr(ref2), None,
+           stream())
+    work = torch.zeros(64 * n2 + 8 * n1, dtype=torch.float64, device=DEV)
+    got1 = torch.zeros_like(ref1)
+    got2 = torch.zeros_like(ref2)
+    g32 = torch.zeros(576 * n2, dtype=torch.float32, device=DEV)
+    L.call("tg_galerkin_from_fine", L.grid(dims), 1, L.ptr(dE), L.ptr(work), L.ptr(got1), None, stream())
+    L.call("tg_galerkin_from_fine", L.grid(dims), 2, L.ptr(dE), L.ptr(work), L.ptr(got2), L.ptr(g32), stream())
+    a1, b1, a2, b2, c32 = h(got1), h(ref1), h(got2), h(ref2), h(g32)
+    assert np.abs(a1 - b1).max() < 1e-12 * np.abs(b1).max()
+    assert np.abs(a2 - b2).max() < 1e-12 * np.abs(b2).max()
+    assert np.array_equal(c32, a2.astype(np.float32))
+
+
 def test_not_spd_is_reported():
     A = t(np.array([1.0, 2.0, 2.0, 1.0]))
     with pytest.raises(L.TopoHipError, match="Cholesky"):

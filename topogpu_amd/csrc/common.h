@@ -86,5 +86,6 @@ extern __constant__ double c_ke64[576];
 extern __constant__ float c_ke32[576];
 extern __constant__ float c_q8_32[8 * 576];
 bool ke_ready();
+const double* ke_host();   // host copy of the unit stiffness given to tg_init
 
 }  // namespace tg

@@ -57,6 +57,9 @@ bool ke_ready() {
   return d >= 0 && d < kMaxDevices && g_ready[d];
 }
 
+double g_ke_host[576];
+const double* ke_host() { return g_ke_host; }
+
 Workspace workspace() {
   const int d = current_device();
   return (d >= 0 && d < kMaxDevices) ? g_ws[d] : Workspace{nullptr, nullptr};
@@ -105,23 +108,32 @@ __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a
     v[0] += ai * (double)b[i];
     if (c) v[1] += ai * (double)c[i];
   }
-  double* const outs[2] = {out, out2};
-  grid_sum<2>(v, outs, ws);
+  block_partials<2>(v, ws);
 }
 
 __global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* a, const double* b,
-                                                    double* out, Workspace ws) {
+                                                    Workspace ws) {
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     v[0] = fma(a[i], b[i], v[0]);
-  double* const outs[1] = {out};
-  grid_sum<1>(v, outs, ws);
+  block_partials<1>(v, ws);
+}
+
+// One block: out_k = sum_b partials[k * nb + b] in block order.
+__global__ void __launch_bounds__(256) k_finalize(int K, int nb, const double* partials, double* out0, double* out1) {
+  for (int k = 0; k < K; ++k) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[(size_t)k * nb + b];
+    block_sum<1>(v);
+    double* o = k == 0 ? out0 : out1;
+    if (threadIdx.x == 0 && o) o[0] = v[0];
+  }
 }
 
 __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, float* r, const float* p,
                                                      const float* q, const double* rz,
-                                                     const double* pq, double* rr, Workspace ws) {
+                                                     const double* pq, Workspace ws) {
   const double den = pq[0];
   const float alpha = (float)(den != 0.0 ? rz[0] / den : 0.0);
   double v[1] = {0.0};
@@ -132,8 +144,7 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, flo
     r[i] = ri;
     v[0] += (double)ri * (double)ri;
   }
-  double* const outs[1] = {rr};
-  grid_sum<1>(v, outs, ws);
+  block_partials<1>(v, ws);
 }
 
 __global__ void k_pcg_dir(long long n, float* p, const float* z, const double* zq, const double* pq) {
@@ -155,7 +166,7 @@ __global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, fl
 }
 
 __global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v, const float* kv, const float* dinv,
-                                                         double* vdv, Workspace ws) {
+                                                         Workspace ws) {
   double acc[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -163,8 +174,7 @@ __global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v,
     acc[0] += vi * vi / (double)dinv[i];
     v[i] = dinv[i] * kv[i];
   }
-  double* const outs[1] = {vdv};
-  grid_sum<1>(acc, outs, ws);
+  block_partials<1>(acc, ws);
 }
 
 __global__ void k_axpy_f32_to_f64(long long n, double* x, const float* d) {
@@ -229,6 +239,7 @@ extern "C" int tg_init(const double* ke) {
         kb[((1 * 2 + t) * 64 + l) * 8 + v] = lo;
       }
 
+  for (int i = 0; i < 576; ++i) g_ke_host[i] = ke[i];
   TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke64), ke, sizeof(double) * 576));
   TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke32), ke32, sizeof(ke32)));
   TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_q8_32), q8, sizeof(q8)));
@@ -267,27 +278,29 @@ extern "C" int tg_dot32(int64_t n, const float* a, const float* b, const float* 
                         double* out2, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && a && b && out && (c == nullptr) == (out2 == nullptr), "bad dot32 args");
-  hipLaunchKernelGGL(k_dot32, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b,
-                     c, out, out2, workspace());
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_dot32, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, c, out, out2,
+                     workspace());
   TG_LAUNCH_CHECK();
-  return TG_OK;
+  return finalize_sums(c ? 2 : 1, nb, out, out2, TG_STREAM(s));
 }
 extern "C" int tg_dot64(int64_t n, const double* a, const double* b, double* out, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && a && b && out, "bad dot64 args");
-  hipLaunchKernelGGL(k_dot64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b,
-                     out, workspace());
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, workspace());
   TG_LAUNCH_CHECK();
-  return TG_OK;
+  return finalize_sums(1, nb, out, nullptr, TG_STREAM(s));
 }
 extern "C" int tg_pcg_xr(int64_t n, float* x, float* r, const float* p, const float* q,
                          const double* rz, const double* pq, double* rr, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && x && r && p && q && rz && pq && rr, "bad pcg_xr args");
-  hipLaunchKernelGGL(k_pcg_xr, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r,
-                     p, q, rz, pq, rr, workspace());
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r, p, q, rz, pq,
+                     workspace());
   TG_LAUNCH_CHECK();
-  return TG_OK;
+  return finalize_sums(1, nb, rr, nullptr, TG_STREAM(s));
 }
 extern "C" int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const double* pq,
                           void* s) {
@@ -316,8 +329,18 @@ extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, fl
 extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && v && kv && dinv && vdv, "bad power_step args");
-  hipLaunchKernelGGL(k_power_step, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
-                     vdv, workspace());
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_power_step, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
+                     workspace());
+  TG_LAUNCH_CHECK();
+  return finalize_sums(1, nb, vdv, nullptr, TG_STREAM(s));
+}
+
+namespace tg {
+int finalize_sums(int K, int nb, double* out0, double* out1, hipStream_t s) {
+  TG_REQUIRE(K >= 1 && K <= 2 && nb >= 1 && (size_t)K * nb <= (size_t)kMaxPartials, "bad reduction shape");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, K, nb, workspace().partials, out0, out1);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
+}  // namespace tg

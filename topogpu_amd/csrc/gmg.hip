@@ -219,6 +219,108 @@ struct SolverHandles {
 };
 SolverHandles g_handles[64];
 
+int get_handle(hipStream_t s, rocblas_handle* out) {
+  int dev = 0;
+  TG_HIP_CHECK(hipGetDevice(&dev));
+  TG_REQUIRE(dev >= 0 && dev < 64, "device id out of range");
+  if (!g_handles[dev].h && rocblas_create_handle(&g_handles[dev].h) != rocblas_status_success)
+    return set_error(TG_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(g_handles[dev].h, s) != rocblas_status_success)
+    return set_error(TG_ERR_HIP, "rocblas_set_stream failed");
+  *out = g_handles[dev].h;
+  return TG_OK;
+}
+
+// Constant Galerkin tables per device (FP64, row-major 576 per matrix):
+// Q[t] = P_t^T KE P_t for the 8 children t of a coarse element, and
+// R[8s+t] = P_s^T Q[t] P_s for the 64 grandchildren two levels down.
+// P_s is the local trilinear prolongation of child s = 4 sx + 2 sy + sz.
+struct GTables {
+  double* q8 = nullptr;
+  double* r64 = nullptr;
+};
+GTables g_tables[64];
+
+void local_prolongation(int s, double P[8][8]) {
+  static const double w1d[3][2] = {{1.0, 0.0}, {0.5, 0.5}, {0.0, 1.0}};
+  const int sx = s >> 2 & 1, sy = s >> 1 & 1, sz = s & 1;
+  for (int c = 0; c < 8; ++c)
+    for (int C = 0; C < 8; ++C)
+      P[c][C] = w1d[sx + corner_x(c)][corner_x(C)] * w1d[sy + corner_y(c)][corner_y(C)] *
+                w1d[sz + corner_z(c)][corner_z(C)];
+}
+
+// out = P^T K P for a 24x24 K and the 8x8 node prolongation P (kron I3).
+void ptkp(const double P[8][8], const double* K, double* out) {
+  for (int A = 0; A < 24; ++A)
+    for (int B = 0; B < 24; ++B) {
+      double acc = 0.0;
+      for (int a = 0; a < 8; ++a) {
+        const double pa = P[a][A / 3];
+        if (pa == 0.0) continue;
+        for (int b = 0; b < 8; ++b) {
+          const double pb = P[b][B / 3];
+          if (pb == 0.0) continue;
+          acc += pa * K[(3 * a + A % 3) * 24 + 3 * b + B % 3] * pb;
+        }
+      }
+      out[A * 24 + B] = acc;
+    }
+}
+
+int ensure_tables(GTables** out) {
+  int dev = 0;
+  TG_HIP_CHECK(hipGetDevice(&dev));
+  TG_REQUIRE(dev >= 0 && dev < 64, "device id out of range");
+  GTables& t = g_tables[dev];
+  if (!t.q8) {
+    static double q8[8 * 576], r64[64 * 576];
+    double P[8][8];
+    for (int s = 0; s < 8; ++s) {
+      local_prolongation(s, P);
+      ptkp(P, ke_host(), q8 + 576 * s);
+    }
+    for (int s = 0; s < 8; ++s) {
+      local_prolongation(s, P);
+      for (int c = 0; c < 8; ++c) ptkp(P, q8 + 576 * c, r64 + 576 * (8 * s + c));
+    }
+    TG_HIP_CHECK(hipMalloc((void**)&t.q8, sizeof(q8)));
+    TG_HIP_CHECK(hipMalloc((void**)&t.r64, sizeof(r64)));
+    TG_HIP_CHECK(hipMemcpy(t.q8, q8, sizeof(q8), hipMemcpyHostToDevice));
+    TG_HIP_CHECK(hipMemcpy(t.r64, r64, sizeof(r64), hipMemcpyHostToDevice));
+  }
+  *out = &t;
+  return TG_OK;
+}
+
+// Ew[C][j] = E of descendant j of coarse element C, `down` levels below:
+// j = t (down = 1) or 8 s + t (down = 2), positions as in the tables above.
+__global__ void k_gather_desc(Grid c, Grid f, int down, const double* __restrict__ fE, double* __restrict__ Ew) {
+  const int m = down == 1 ? 8 : 64;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= c.elems() * m) return;
+  const long long C = t / m;
+  const int j = (int)(t % m);
+  const int K = (int)(C % c.nz), J = (int)((C / c.nz) % c.ny), I = (int)(C / ((long long)c.nz * c.ny));
+  int fi, fj, fk;
+  if (down == 1) {
+    fi = 2 * I + (j >> 2 & 1);
+    fj = 2 * J + (j >> 1 & 1);
+    fk = 2 * K + (j & 1);
+  } else {
+    const int s = j >> 3, u = j & 7;
+    fi = 4 * I + 2 * (s >> 2 & 1) + (u >> 2 & 1);
+    fj = 4 * J + 2 * (s >> 1 & 1) + (u >> 1 & 1);
+    fk = 4 * K + 2 * (s & 1) + (u & 1);
+  }
+  Ew[t] = fE[f.elem(fi, fj, fk)];
+}
+
+__global__ void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 }  // namespace
 }  // namespace tg
 
@@ -287,16 +389,8 @@ extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const in
 
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
-  int dev = 0;
-  TG_HIP_CHECK(hipGetDevice(&dev));
-  TG_REQUIRE(dev >= 0 && dev < 64, "device id out of range");
-  if (!g_handles[dev].h) {
-    if (rocblas_create_handle(&g_handles[dev].h) != rocblas_status_success)
-      return set_error(TG_ERR_HIP, "rocblas_create_handle failed");
-  }
-  rocblas_handle h = g_handles[dev].h;
-  if (rocblas_set_stream(h, (hipStream_t)s) != rocblas_status_success)
-    return set_error(TG_ERR_HIP, "rocblas_set_stream failed");
+  rocblas_handle h = nullptr;
+  if (int r = get_handle((hipStream_t)s, &h)) return r;
   rocblas_int* dinfo = nullptr;
   TG_HIP_CHECK(hipMallocAsync((void**)&dinfo, sizeof(rocblas_int), (hipStream_t)s));
   // Row-major symmetric A is its own column-major transpose; work on the
@@ -331,5 +425,41 @@ extern "C" int tg_coarsest_apply(const double* Ainv, int32_t n, const int32_t* f
   hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)s, Ainv, n, fl, rc,
                      ec);
   TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_galerkin_from_fine(const tg_grid* fine, int down, const double* fine_E, double* work,
+                                     double* out64, float* out32, void* s) {
+  TG_NEED_INIT();
+  if (int r = check_grid(fine)) return r;
+  TG_REQUIRE(down == 1 || down == 2, "levels_down must be 1 or 2");
+  TG_REQUIRE(fine_E && work && out64, "galerkin_from_fine: NULL argument");
+  const int f2 = 1 << down;
+  TG_REQUIRE(fine->nelx % f2 == 0 && fine->nely % f2 == 0 && fine->nelz % f2 == 0,
+             "galerkin_from_fine: fine dims must be divisible by %d", f2);
+  const Grid f = to_grid(fine);
+  const Grid c{f.nx / f2, f.ny / f2, f.nz / f2};
+  GTables* tab = nullptr;
+  if (int r = ensure_tables(&tab)) return r;
+  const int m = down == 1 ? 8 : 64;
+  const long long nw = c.elems() * m;
+  hipLaunchKernelGGL(k_gather_desc, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, (hipStream_t)s, c, f, down,
+                     fine_E, work);
+  TG_LAUNCH_CHECK();
+  // Row-major K (nc x 576) = Ew (nc x m) * T (m x 576); in column-major terms
+  // K^T = T^T * Ew^T with the row-major arrays read as their transposes.
+  rocblas_handle h = nullptr;
+  if (int r = get_handle((hipStream_t)s, &h)) return r;
+  const double one = 1.0, zero = 0.0;
+  TG_REQUIRE(c.elems() < (1LL << 31), "too many coarse elements");
+  const rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, 576, (rocblas_int)c.elems(),
+                                          m, &one, down == 1 ? tab->q8 : tab->r64, 576, work, m, &zero, out64, 576);
+  if (st != rocblas_status_success) return set_error(TG_ERR_HIP, "rocblas_dgemm status %d", (int)st);
+  if (out32) {
+    const long long n = 576 * c.elems();
+    const long long nb = (n + 255) / 256 > 4096 ? 4096 : (n + 255) / 256;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)s, n, out64, out32);
+    TG_LAUNCH_CHECK();
+  }
   return TG_OK;
 }

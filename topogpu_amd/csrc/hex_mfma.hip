@@ -56,7 +56,10 @@ __global__ void __launch_bounds__(NTHR) k_hex_mfma(Grid g, const uint8_t* __rest
                                                   const float* __restrict__ E, const float* __restrict__ u,
                                                   Ep ep, double* dot, Workspace ws) {
   __shared__ float4 su[UBX * UBY * UBZ];
-  __shared__ float slot[MTX * MTY][MTZ][4][3];
+  // [line][corner][z][3]: consecutive lanes touch consecutive 3-float
+  // records (odd stride in 4-byte banks), so writes and reads are
+  // free of bank conflicts.
+  __shared__ float slot[MTX * MTY][4][MTZ][3];
 
   const int ntz = (g.NZ() + MTZ - 1) / MTZ, nty = (g.NY() + MTY - 1) / MTY;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -64,19 +67,29 @@ __global__ void __launch_bounds__(NTHR) k_hex_mfma(Grid g, const uint8_t* __rest
   const int j0 = ((t / ntz) % nty) * MTY;
   const int i0 = (t / (ntz * nty)) * MTX;
 
-  for (int idx = threadIdx.x; idx < UBX * UBY * UBZ; idx += NTHR) {
+  // Node brick: every load is issued before any is used (no load depends on
+  // another), so a workgroup pays one memory round trip, not one per node.
+  constexpr int NB = UBX * UBY * UBZ, NIT = (NB + NTHR - 1) / NTHR;
+  float vx[NIT], vy[NIT], vz[NIT];
+  unsigned vm[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = threadIdx.x + it * NTHR;
     const int bz = idx % UBZ, by = (idx / UBZ) % UBY, bx = idx / (UBZ * UBY);
     const int i = i0 - 1 + bx, j = j0 - 1 + by, k = k0 - 1 + bz;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz) {
-      const long long n = g.node(i, j, k);
-      const unsigned m = mask[n];
-      const float* p = u + 3 * n;
-      v.x = (m & 1u) ? p[0] : 0.f;
-      v.y = (m & 2u) ? p[1] : 0.f;
-      v.z = (m & 4u) ? p[2] : 0.f;
-    }
-    su[idx] = v;
+    const bool in = idx < NB && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+    const long long n = in ? g.node(i, j, k) : 0;
+    vm[it] = in ? (unsigned)mask[n] : 0u;
+    vx[it] = in ? u[3 * n] : 0.f;
+    vy[it] = in ? u[3 * n + 1] : 0.f;
+    vz[it] = in ? u[3 * n + 2] : 0.f;
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = threadIdx.x + it * NTHR;
+    if (idx < NB)
+      su[idx] = make_float4((vm[it] & 1u) ? vx[it] : 0.f, (vm[it] & 2u) ? vy[it] : 0.f,
+                            (vm[it] & 4u) ? vz[it] : 0.f, 0.f);
   }
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -131,29 +144,44 @@ __global__ void __launch_bounds__(NTHR) k_hex_mfma(Grid g, const uint8_t* __rest
     const int ti = px - 1 + corner_x(grp), tj = py - 1 + corner_y(grp);
     if (je >= 1 && ti >= 0 && ti < MTX && tj >= 0 && tj < MTY) {
 #pragma unroll
-      for (int v = 0; v < 3; ++v) slot[ti * MTY + tj][je - 1][grp][v] = s[v];
+      for (int v = 0; v < 3; ++v) slot[ti * MTY + tj][grp][je - 1][v] = s[v];
     }
   }
   __syncthreads();
 
-  double part = 0.0;
-  for (int idx = threadIdx.x; idx < MTX * MTY * MTZ; idx += NTHR) {
+  // Output: all epilogue loads of this thread's nodes first, then the stores.
+  constexpr int NOUT = MTX * MTY * MTZ, OIT = (NOUT + NTHR - 1) / NTHR;
+  typename Ep::In in[OIT];
+  long long nn[OIT];
+  unsigned mo[OIT];
+  bool ok[OIT];
+#pragma unroll
+  for (int it = 0; it < OIT; ++it) {
+    const int idx = threadIdx.x + it * NTHR;
     const int z = idx % MTZ, line = idx / MTZ, ti = line / MTY, tj = line % MTY;
     const int i = i0 + ti, j = j0 + tj, k = k0 + z;
-    if (i > g.nx || j > g.ny || k > g.nz) continue;
+    ok[it] = idx < NOUT && i <= g.nx && j <= g.ny && k <= g.nz;
+    nn[it] = ok[it] ? g.node(i, j, k) : 0;
+    mo[it] = ok[it] ? (unsigned)mask[nn[it]] : 0u;
+    if (ok[it]) ep.load(nn[it], in[it]);
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int it = 0; it < OIT; ++it) {
+    if (!ok[it]) continue;
+    const int idx = threadIdx.x + it * NTHR;
+    const int z = idx % MTZ, line = idx / MTZ, ti = line / MTY, tj = line % MTY;
     float au[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v)
-      au[v] = ((slot[line][z][0][v] + slot[line][z][1][v]) + slot[line][z][2][v]) + slot[line][z][3][v];
+      au[v] = ((slot[line][0][z][v] + slot[line][1][z][v]) + slot[line][2][z][v]) + slot[line][3][z][v];
     const float4 uo = su[((ti + 1) * UBY + tj + 1) * UBZ + z + 1];
     const float own[3] = {uo.x, uo.y, uo.z};
-    const long long n = g.node(i, j, k);
-    part += ep(n, mask[n], au, own);
+    part += ep.store(nn[it], mo[it], au, own, in[it]);
   }
   if (ep.reduces()) {
     double v[1] = {part};
-    double* const outs[1] = {dot};
-    grid_sum<1>(v, outs, ws);
+    block_partials<1>(v, ws);
   }
 }
 
@@ -187,7 +215,7 @@ int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const fl
     hipLaunchKernelGGL((k_hex_mfma<false, Ep32>), dim3(nb), dim3(NTHR), 0, s, g, mask, E, u, ep, dot,
                        workspace());
   TG_LAUNCH_CHECK();
-  return TG_OK;
+  return ep.reduces() ? finalize_sums(1, nb, dot, nullptr, s) : TG_OK;
 }
 
 }  // namespace tg

@@ -67,21 +67,27 @@ template <typename T>
 __device__ __forceinline__ void load_node_brick(const Grid& g, const uint8_t* __restrict__ mask,
                                                 const T* __restrict__ u, int i0, int j0, int k0,
                                                 T* su) {
-  for (int idx = threadIdx.x; idx < BX * BY * BZ; idx += blockDim.x) {
+  // All loads first (none depends on another), then the masked LDS stores.
+  constexpr int NB = BX * BY * BZ, NIT = (NB + NT - 1) / NT;
+  T v[NIT][3];
+  unsigned vm[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = threadIdx.x + it * NT;
     const int bz = idx % BZ, by = (idx / BZ) % BY, bx = idx / (BZ * BY);
     const int i = i0 - 1 + bx, j = j0 - 1 + by, k = k0 - 1 + bz;
-    T v0 = 0, v1 = 0, v2 = 0;
-    if (i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz) {
-      const long long n = g.node(i, j, k);
-      const unsigned m = mask[n];
-      const T* p = u + 3 * n;
-      v0 = (m & 1u) ? p[0] : T(0);
-      v1 = (m & 2u) ? p[1] : T(0);
-      v2 = (m & 4u) ? p[2] : T(0);
-    }
-    su[3 * idx + 0] = v0;
-    su[3 * idx + 1] = v1;
-    su[3 * idx + 2] = v2;
+    const bool in = idx < NB && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+    const long long n = in ? g.node(i, j, k) : 0;
+    vm[it] = in ? (unsigned)mask[n] : 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[it][c] = in ? u[3 * n + c] : T(0);
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = threadIdx.x + it * NT;
+    if (idx < NB)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) su[3 * idx + c] = (vm[it] >> c & 1u) ? v[it][c] : T(0);
   }
 }
 
@@ -108,6 +114,11 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
   const int tz = threadIdx.x % NTZ, ty = (threadIdx.x / NTZ) % NTY, tx = threadIdx.x / (NTZ * NTY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   const bool own = i <= g.nx && j <= g.ny && k <= g.nz;
+  // epilogue inputs are fetched now so their latency hides behind the FMAs
+  const long long n_own = own ? g.node(i, j, k) : 0;
+  const unsigned m_own = own ? (unsigned)mask[n_own] : 0u;
+  typename Ep::In ein;
+  if (own) ep.load(n_own, ein);
   double part = 0.0;
   if (own) {
     T nb[27][3];
@@ -144,14 +155,12 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
         acc[2] = fma(we, t2, acc[2]);
       }
     }
-    const long long n = g.node(i, j, k);
     const T own_u[3] = {nb[13][0], nb[13][1], nb[13][2]};
-    part = ep(n, mask[n], acc, own_u);
+    part = ep.store(n_own, m_own, acc, own_u, ein);
   }
   if (ep.reduces()) {
     double v[1] = {part};
-    double* const outs[1] = {dot};
-    grid_sum<1>(v, outs, ws);
+    block_partials<1>(v, ws);
   }
 }
 
@@ -204,8 +213,7 @@ __global__ void __launch_bounds__(NT) k_node_stored(Grid g, const uint8_t* __res
   }
   if (ep.reduces()) {
     double v[1] = {part};
-    double* const outs[1] = {dot};
-    grid_sum<1>(v, outs, ws);
+    block_partials<1>(v, ws);
   }
 }
 
@@ -253,7 +261,7 @@ int launch_node(const Grid& g, const uint8_t* mask, W w, const T* u, const Ep& e
   TG_REQUIRE(nb <= kMaxPartials, "grid too large for the reduction workspace");
   hipLaunchKernelGGL((k_node_op<T, W, Ep>), dim3(nb), dim3(NT), 0, s, g, mask, w, u, ep, dot, workspace());
   TG_LAUNCH_CHECK();
-  return TG_OK;
+  return ep.reduces() ? finalize_sums(1, nb, dot, nullptr, s) : TG_OK;
 }
 
 }  // namespace
@@ -310,7 +318,7 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
       hipLaunchKernelGGL((k_node_stored<Ep32>), dim3(nb), dim3(NT), 0, s, g, op->mask, op->elem_mats, u, e,
                          ep->dot, workspace());
       TG_LAUNCH_CHECK();
-      return TG_OK;
+      return e.reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
     }
     default:
       return set_error(TG_ERR_ARG, "unknown operator kind %d", op->kind);

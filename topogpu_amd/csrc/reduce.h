@@ -1,10 +1,12 @@
-// Deterministic FP64 reductions across a grid without a second launch.
+// Deterministic FP64 reductions across a grid.
 //
-// Each block reduces its values in a fixed tree, writes one partial per
-// output to the library workspace and takes a ticket; the block holding the
-// last ticket sums all partials in block order and writes the result. The
-// summation order depends only on the launch shape, so results are
-// bit-reproducible run to run (unlike atomic accumulation).
+// A reducing kernel writes one partial per block (fixed tree inside the
+// block) to the library workspace; a one-block finalize kernel launched
+// right after it on the same stream sums the partials in block order. The
+// kernel boundary makes the partials visible, so no device-scope fence is
+// needed inside the hot kernel (on MI355X such a fence writes back the
+// XCD's L2 and costs more than the reduction itself). The summation order
+// depends only on the launch shape: results are bit-reproducible.
 #pragma once
 
 #include "common.h"
@@ -13,7 +15,7 @@ namespace tg {
 
 struct Workspace {
   double* partials;     // >= kMaxPartials doubles
-  unsigned* ticket;     // one counter, returned to 0 by the last block
+  unsigned* ticket;     // spare counter word (kept zero)
 };
 constexpr int kMaxPartials = 1 << 19;   // 4 MiB of FP64 partials
 Workspace workspace();                  // per current device, valid after tg_init
@@ -48,44 +50,21 @@ __device__ __forceinline__ void block_sum(double (&v)[K]) {
   __syncthreads();
 }
 
-// Grid-wide sum of K per-thread values into out[k] (device pointers; an
-// out[k] may be nullptr). Must be reached by every thread of every block.
-// Returns true in the block that performed the final sum (after out[] is
-// written), which can then run follow-up single-block logic.
+// Write this block's K partial sums to ws.partials[k * nblocks + block].
+// Must be reached by every thread of every block.
 template <int K>
-__device__ bool grid_sum(double (&v)[K], double* const (&out)[K], Workspace ws) {
-  __shared__ bool s_last;
+__device__ __forceinline__ void block_partials(double (&v)[K], Workspace ws) {
   block_sum<K>(v);
-  const int nb = gridDim.x * gridDim.y * gridDim.z;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (threadIdx.x == 0) {
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for (int k = 0; k < K; ++k) ws.partials[(size_t)k * nb + bid] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(ws.ticket, 1u);
-    s_last = (t == (unsigned)(nb - 1));
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      s += ws.partials[(size_t)k * nb + b];
-    acc[k] = s;
-  }
-  block_sum<K>(acc);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (out[k]) out[k][0] = acc[k];
-    *ws.ticket = 0u;
-    __threadfence();
-  }
-  __syncthreads();
-  return true;
 }
+
+// Host side: launch the one-block finalize that sums K x nblocks partials
+// into out[0..K) (entries may be nullptr). Defined in core.hip.
+int finalize_sums(int K, int nblocks, double* out0, double* out1, hipStream_t s);
 
 }  // namespace tg

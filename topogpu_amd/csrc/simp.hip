@@ -109,7 +109,7 @@ __global__ void k_sensitivity(Grid g, const double* __restrict__ U, const double
 
 __device__ __forceinline__ double oc_candidate(double q, double lo, double hi, double lm) {
   // pub_simp_solver.py:326-328: clip to [rho-move, rho+move], then [1e-3, 1]
-  const double x = q / sqrt(lm);
+  const double x = q * (1.0 / sqrt(lm));   // same arithmetic as k_oc_pass
   return fmin(fmax(fmin(fmax(x, lo), hi), 1e-3), 1.0);
 }
 
@@ -132,42 +132,103 @@ __global__ void k_oc_prepare(long long n, const double* __restrict__ rho, const 
   hi[e] = rho[e] + move;
 }
 
-__global__ void __launch_bounds__(256) k_oc_step(Grid g, Stencil st, const double* __restrict__ hs_inv,
+// One OC pass covers OC_LEVELS bisection steps at once. The 2^L - 1 candidate
+// multipliers are the midpoints of the binary tree below the current bracket
+// [l1, l2], each computed as 0.5*(a+b) of its own bracket -- the same
+// arithmetic the sequential bisection performs on whichever path it takes.
+// The filtered/projected volume is monotone in lambda, so walking the tree
+// with the evaluated volumes reproduces the sequential decisions exactly.
+constexpr int OC_LEVELS = 4;
+constexpr int OC_NPTS = (1 << OC_LEVELS) - 1;   // tree nodes in heap order
+
+__device__ __forceinline__ void oc_tree(double l1, double l2, double (&lam)[OC_NPTS]) {
+  double lo_[OC_NPTS], hi_[OC_NPTS];
+  lo_[0] = l1;
+  hi_[0] = l2;
+#pragma unroll
+  for (int t = 0; t < OC_NPTS; ++t) {
+    lam[t] = 0.5 * (lo_[t] + hi_[t]);
+    if (2 * t + 2 < OC_NPTS) {
+      lo_[2 * t + 1] = lo_[t];  hi_[2 * t + 1] = lam[t];   // volume too small -> left
+      lo_[2 * t + 2] = lam[t];  hi_[2 * t + 2] = hi_[t];   // volume too large -> right
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_oc_pass(Grid g, Stencil st, const double* __restrict__ hs_inv,
                                                 const double* __restrict__ q, const double* __restrict__ lo,
                                                 const double* __restrict__ hi, double beta, double eta,
-                                                double volfrac, int use_h, double* state, Workspace ws) {
+                                                int use_h, const double* state, Workspace ws) {
   if (state[3] != 0.0) return;   // uniform: converged bracket
-  const double lm = 0.5 * (state[0] + state[1]);
-  double v[1] = {0.0};
+  double lam[OC_NPTS], rs[OC_NPTS];
+  oc_tree(state[0], state[1], lam);
+  // 1/sqrt(lambda) once per candidate: the element loop then multiplies
+  // (an FP64 divide per neighbour and candidate would dominate the pass)
+#pragma unroll
+  for (int t = 0; t < OC_NPTS; ++t) rs[t] = 1.0 / sqrt(lam[t]);
+  double v[OC_NPTS];
+#pragma unroll
+  for (int t = 0; t < OC_NPTS; ++t) v[t] = 0.0;
+  const bool proj_on = use_h && beta > 1e-6;
   const long long n = g.elems();
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
     const int k = (int)(e % g.nz), j = (int)((e / g.nz) % g.ny), i = (int)(e / ((long long)g.nz * g.ny));
-    double s = 0.0;
-    for (int t = 0; t < st.m; ++t) {
-      const int a = i + st.off[t][0], b = j + st.off[t][1], c = k + st.off[t][2];
+    double s[OC_NPTS];
+#pragma unroll
+    for (int t = 0; t < OC_NPTS; ++t) s[t] = 0.0;
+    for (int o = 0; o < st.m; ++o) {
+      const int a = i + st.off[o][0], b = j + st.off[o][1], c = k + st.off[o][2];
       if (a < 0 || a >= g.nx || b < 0 || b >= g.ny || c < 0 || c >= g.nz) continue;
       const long long f = g.elem(a, b, c);
-      s = fma(st.w[t], oc_candidate(q[f], lo[f], hi[f], lm), s);
+      const double qf = q[f], lf = lo[f], hf = hi[f], w = st.w[o];
+#pragma unroll
+      for (int t = 0; t < OC_NPTS; ++t)
+        s[t] = fma(w, fmin(fmax(fmin(fmax(qf * rs[t], lf), hf), 1e-3), 1.0), s[t]);
     }
-    const double rf = s * hs_inv[e];
-    v[0] += (use_h && beta > 1e-6) ? proj(rf, beta, eta) : rf;
+    const double hinv = hs_inv[e];
+#pragma unroll
+    for (int t = 0; t < OC_NPTS; ++t) {
+      const double rf = s[t] * hinv;
+      v[t] += proj_on ? proj(rf, beta, eta) : rf;
+    }
   }
-  double vol = 0.0;
-  double* const outs[1] = {&state[5]};
-  if (grid_sum<1>(v, outs, ws) && threadIdx.x == 0) {
-    vol = state[5] / (double)n;
-    if (vol > volfrac) state[0] = lm; else state[1] = lm;
-    state[2] = lm;
-    state[4] += 1.0;
-    if (state[1] - state[0] < 1e-9) state[3] = 1.0;
-    __threadfence();
+  block_partials<OC_NPTS>(v, ws);
+}
+
+// One block: the volumes of the pass's candidates, then up to OC_LEVELS
+// bracket updates of pub_simp_solver.py:333-338, stopping when l2-l1 < 1e-9.
+__global__ void __launch_bounds__(256) k_oc_walk(int nb, long long n, double volfrac, int max_steps, double* state,
+                                                Workspace ws) {
+  if (state[3] != 0.0) return;
+  __shared__ double svol[OC_NPTS];
+  for (int t = 0; t < OC_NPTS; ++t) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ws.partials[(size_t)t * nb + b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) svol[t] = v[0];
+  }
+  if (threadIdx.x == 0) {
+    double lam[OC_NPTS];
+    oc_tree(state[0], state[1], lam);
+    int node = 0;
+    for (int lev = 0; lev < OC_LEVELS; ++lev) {
+      if (state[4] >= max_steps) { state[3] = 1.0; break; }
+      const double lm = lam[node];
+      state[5] = svol[node];
+      const bool right = svol[node] / (double)n > volfrac;
+      if (right) state[0] = lm; else state[1] = lm;
+      state[2] = lm;
+      state[4] += 1.0;
+      if (state[1] - state[0] < 1e-9 || state[4] >= max_steps) { state[3] = 1.0; break; }
+      node = 2 * node + (right ? 2 : 1);
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) k_oc_finalize(long long n, const double* __restrict__ rho,
                                                     const double* __restrict__ q, const double* __restrict__ lo,
                                                     const double* __restrict__ hi, const double* state,
-                                                    double* rho_new, double* change, Workspace ws) {
+                                                    double* rho_new, Workspace ws) {
   const double lm = state[2];
   double mx = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
@@ -175,7 +236,6 @@ __global__ void __launch_bounds__(256) k_oc_finalize(long long n, const double* 
     rho_new[e] = r;
     mx = fmax(mx, fabs(r - rho[e]));
   }
-  // max via the deterministic sum machinery on a per-block max
   __shared__ double smax[4];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -185,17 +245,19 @@ __global__ void __launch_bounds__(256) k_oc_finalize(long long n, const double* 
     double b = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, smax[w]);
     ws.partials[blockIdx.x] = b;
-    __threadfence();
-    const unsigned t = atomicAdd(ws.ticket, 1u);
-    if (t == gridDim.x - 1) {
-      __threadfence();
-      double r = 0.0;
-      for (unsigned k = 0; k < gridDim.x; ++k) r = fmax(r, ws.partials[k]);
-      *change = r;
-      *ws.ticket = 0u;
-      __threadfence();
-    }
   }
+}
+
+// One block: max over the per-block maxima (order-independent).
+__global__ void __launch_bounds__(256) k_max_finalize(int nb, double* out, Workspace ws) {
+  __shared__ double smax[4];
+  double mx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) mx = fmax(mx, ws.partials[b]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 }
 
 int grid_blocks(long long n) {
@@ -264,8 +326,12 @@ extern "C" int tg_oc_bisect_step(const tg_grid* gp, int32_t m, const int32_t* of
   Stencil st;
   if (int r = make_stencil(m, offs, w, st)) return r;
   const Grid g = to_grid(gp);
-  hipLaunchKernelGGL(k_oc_step, dim3(grid_blocks(g.elems())), dim3(256), 0, (hipStream_t)s, g, st, hs_inv, q, lo,
-                     hi, beta, eta, volfrac, use_h, state, workspace());
+  const int nb = grid_blocks(g.elems());
+  hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(256), 0, (hipStream_t)s, g, st, hs_inv, q, lo, hi, beta, eta, use_h,
+                     state, workspace());
+  TG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_oc_walk, dim3(1), dim3(256), 0, (hipStream_t)s, nb, g.elems(), volfrac, 100, state,
+                     workspace());
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
@@ -274,8 +340,11 @@ extern "C" int tg_oc_finalize(int64_t n, const double* rho, const double* q, con
                               const double* state, double* rho_new, double* change, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && rho && q && lo && hi && state && rho_new && change, "oc_finalize: NULL argument");
-  hipLaunchKernelGGL(k_oc_finalize, dim3(grid_blocks(n)), dim3(256), 0, (hipStream_t)s, (long long)n, rho, q, lo, hi,
-                     state, rho_new, change, workspace());
+  const int nb = grid_blocks(n);
+  hipLaunchKernelGGL(k_oc_finalize, dim3(nb), dim3(256), 0, (hipStream_t)s, (long long)n, rho, q, lo, hi, state,
+                     rho_new, workspace());
+  TG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_finalize, dim3(1), dim3(256), 0, (hipStream_t)s, nb, change, workspace());
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

@@ -66,6 +66,22 @@ def level_dims(dims, n_levels: int):
     return out
 
 
+def auto_levels(dims, mask0: np.ndarray, max_coarse: int = 2000, max_levels: int = 10) -> int:
+    """Fewest levels (>= 2) whose coarsest level has at most ``max_coarse``
+    free DOFs. The coarsest level is factorised densely once per design
+    iteration, at O(n^3) cost, so its size, not the depth of the hierarchy,
+    bounds the setup time."""
+    dims_l, mask = tuple(dims), mask0
+    for n_levels in range(2, max_levels + 1):
+        nxt = coarse_dims(dims_l)
+        mask = injection_mask(dims_l, nxt, mask)
+        dims_l = nxt
+        n_free = int(((mask[:, None] >> np.arange(3)[None, :]) & 1).sum())
+        if n_free <= max_coarse or dims_l == (1, 1, 1):
+            return n_levels
+    return max_levels
+
+
 def injection_mask(dims_f, dims_c, mask_f: np.ndarray) -> np.ndarray:
     """Coarse node mask by injection: coarse node (I,J,K) copies the mask of
     fine node (2I, 2J, 2K) (solver_v2.py:717-764). On one-element-thick axes

@@ -116,6 +116,19 @@ class GMG:
             l1.kind = L.TG_OP_Q8_F32
         for lv in range(1, n_levels):
             self.levels[lv].kind_accurate = self.levels[lv].kind
+        # Galerkin build plan: aligned levels come straight from the fine E by
+        # one gather + DGEMM (levels 1 and 2); the rest from the level above.
+        even = lambda d: all(n % 2 == 0 for n in d)   # noqa: E731
+        self.l1_from_fine = even(self.dims[0]) and l1.consistent
+        self.l2_from_fine = (n_levels > 2 and self.l1_from_fine and even(self.dims[1])
+                             and self.levels[2].consistent)
+        self.need_l1_mats = l1.kind != L.TG_OP_Q8_F32 or n_levels == 2 or not self.l2_from_fine
+        work = 0
+        if self.l1_from_fine and self.need_l1_mats:
+            work = max(work, 8 * self.levels[1].n_elem)
+        if self.l2_from_fine:
+            work = max(work, 64 * self.levels[2].n_elem)
+        self.gwork = torch.empty(max(work, 1), dtype=torch.float64, device=device)
         last = self.levels[-1]
         free = fem.free_from_node_mask(last.mask_host)
         self.n_coarse = int(free.size)
@@ -155,14 +168,26 @@ class GMG:
         last_lv = len(self.levels) - 1
         for lv in range(1, len(self.levels)):
             f, c = self.levels[lv - 1], self.levels[lv]
-            if c.mats64 is None:
-                c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
-            need32 = c.kind == L.TG_OP_STORED_F32 and lv < last_lv
-            if need32 and c.mats32 is None:
-                c.mats32 = torch.empty(576 * c.n_elem, dtype=torch.float32, device=self.dev)
-            L.call("tg_galerkin_elems", f.grid, c.grid, L.ptr(f.mask), int(not c.consistent),
-                   L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64), L.ptr(c.mats64),
-                   L.ptr(c.mats32) if need32 else None, stream)
+            if lv == 1 and not self.need_l1_mats:
+                pass                                    # applied implicitly (Q8)
+            else:
+                if c.mats64 is None:
+                    c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
+                need32 = c.kind == L.TG_OP_STORED_F32 and lv < last_lv
+                if need32 and c.mats32 is None:
+                    c.mats32 = torch.empty(576 * c.n_elem, dtype=torch.float32, device=self.dev)
+                m32 = L.ptr(c.mats32) if need32 else None
+                lv0 = self.levels[0]
+                if lv == 1 and self.l1_from_fine:
+                    L.call("tg_galerkin_from_fine", lv0.grid, 1, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
+                           m32, stream)
+                elif lv == 2 and self.l2_from_fine:
+                    L.call("tg_galerkin_from_fine", lv0.grid, 2, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
+                           m32, stream)
+                else:
+                    L.call("tg_galerkin_elems", f.grid, c.grid, L.ptr(f.mask), int(not c.consistent),
+                           L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64),
+                           L.ptr(c.mats64), m32, stream)
             if lv < last_lv:
                 if c.kind == L.TG_OP_Q8_F32:
                     L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)

@@ -54,6 +54,7 @@ _SIGS = {
     "tg_restrict32": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_prolong32": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, _vp, _vp, C.c_int, _vp],
     "tg_galerkin_elems": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, C.c_int, _vp, _vp, _vp, _vp, _vp],
+    "tg_galerkin_from_fine": [C.POINTER(TgGrid), C.c_int, _vp, _vp, _vp, _vp, _vp],
     "tg_assemble_dense": [C.POINTER(TgGrid), _vp, _vp, _i32, _vp, _vp],
     "tg_dense_spd_inverse": [_vp, _i32, _vp],
     "tg_coarsest_apply": [_vp, _i32, _vp, _i64, _vp, _vp, _vp],

@@ -119,13 +119,15 @@ class HipSIMP:
         use_h = int(self.use_heaviside and self.beta > 1e-6)
         done = 0
         steps = 0
+        # each call advances up to 4 bisection steps; a [0, 1e9] bracket
+        # needs 60 steps to shrink below 1e-9, i.e. 15 calls
         while steps < max_steps:
-            chunk = min(64 if steps == 0 else 8, max_steps - steps)
+            chunk = 15 if steps == 0 else 2
             for _ in range(chunk):
                 L.call("tg_oc_bisect_step", self.grid, self.m, self.offs.ctypes.data, self.w.ctypes.data,
                        L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
                        self.volfrac, use_h, L.ptr(self.oc_state), st)
-            steps += chunk
+            steps += 4 * chunk
             done = float(self.oc_state[3].item())
             if done:
                 break

@@ -77,6 +77,8 @@ class HipStateSolver:
         self.n_elem = int(np.prod(self.dims))
         mask = fem.node_mask_from_free(free, self.n_nodes)
         self.mask = torch.from_numpy(mask).to(self.dev)
+        if n_levels == "auto":
+            n_levels = fem.auto_levels(self.dims, mask)
         F = np.asarray(F, dtype=np.float64).copy()
         F[np.setdiff1d(np.arange(self.n), free)] = 0.0
         self.F = torch.from_numpy(F).to(self.dev)
