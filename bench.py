@@ -251,6 +251,7 @@ def main() -> int:
     ap.add_argument("--no-matvec", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the auto-depth hierarchy runs")
     ap.add_argument("--levels", default=None, help="override the hierarchy depth of the headline run (int or auto)")
+    ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode (see DESIGN.md for how it would "
@@ -261,6 +262,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
+    if args.matvec_only:
+        print(json.dumps({"matvec": matvec_bench()}), flush=True)
+        return 0
 
     levels = None if args.levels is None else (args.levels if args.levels == "auto" else int(args.levels))
     bf, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, timed_kernels=True, n_levels=levels)

@@ -7,12 +7,19 @@
 // mode needs (unconditionally, independent of the mask), store() combines,
 // applies the mask and writes, returning the node's contribution to the
 // optional grid reduction.
+//
+// The mode is a template parameter: each kernel is instantiated per mode, so
+// the per-node code holds only that mode's loads, arithmetic and stores. The
+// formulas are written once against an accessor that maps (vector, component)
+// to memory; PtrAcc addresses plain global memory, the MFMA operator uses
+// buffer descriptors with 32-bit offsets (hex_mfma.hip).
 #pragma once
 
 #include "common.h"
 
 namespace tg {
 
+// Runtime arguments of the FP32 epilogue (tg_ep32 plus the output y).
 struct Ep32 {
   int mode;
   const float* rhs;
@@ -22,99 +29,130 @@ struct Ep32 {
   float* r;
   float a1, a2, a3;
   float* y;
+};
 
+enum EpVec { EV_RHS = 0, EV_DINV, EV_D, EV_X, EV_R, EV_Y, EV_COUNT };
+
+template <int MODE>
+struct EpMode {
   struct In {
     float p[3], q[3], s[3];
   };
+  static constexpr bool kReduces = MODE == TG_EP_DOT || MODE == TG_EP_RESID_NORM;
 
-  __device__ __forceinline__ void load(long long node, In& in) const {
-    const long long b = 3 * node;
+  template <class Acc>
+  __device__ __forceinline__ static void load(const Ep32& e, const Acc& a, In& in) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       in.p[c] = in.q[c] = in.s[c] = 0.0f;
-      switch (mode) {
-        case TG_EP_RESID:
-        case TG_EP_RESID_NORM:
-          in.p[c] = rhs[b + c];
-          break;
-        case TG_EP_CHEB:
-          in.p[c] = rhs[b + c];
-          in.q[c] = dinv[b + c];
-          if (a2 != 0.0f) in.s[c] = d[b + c];
-          break;
-        case TG_EP_CHEB_R:
-          in.p[c] = r[b + c];
-          in.q[c] = dinv[b + c];
-          in.s[c] = x_out[b + c];
-          break;
-        case TG_EP_RESID_UPDATE:
-          in.p[c] = r[b + c];
-          break;
-        default:
-          break;
+      if constexpr (MODE == TG_EP_RESID || MODE == TG_EP_RESID_NORM) {
+        in.p[c] = a.template ld<EV_RHS>(c);
+      } else if constexpr (MODE == TG_EP_CHEB) {
+        in.p[c] = a.template ld<EV_RHS>(c);
+        in.q[c] = a.template ld<EV_DINV>(c);
+        if (e.a2 != 0.0f) in.s[c] = a.template ld<EV_D>(c);
+      } else if constexpr (MODE == TG_EP_CHEB_R) {
+        in.p[c] = a.template ld<EV_R>(c);
+        in.q[c] = a.template ld<EV_DINV>(c);
+        in.s[c] = a.template ld<EV_X>(c);
+      } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
+        in.p[c] = a.template ld<EV_R>(c);
       }
     }
   }
 
-  __device__ __forceinline__ double store(long long node, unsigned m, const float (&au)[3], const float (&u)[3],
-                                          const In& in) const {
-    const long long b = 3 * node;
+  template <class Acc>
+  __device__ __forceinline__ static double store(const Ep32& e, const Acc& a, unsigned m, const float (&au)[3],
+                                                 const float (&u)[3], const In& in) {
     double part = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const bool f = (m >> c) & 1u;
-      switch (mode) {
-        case TG_EP_STORE:
-          y[b + c] = f ? au[c] : 0.0f;
-          break;
-        case TG_EP_DOT: {
-          const float v = f ? au[c] : 0.0f;
-          y[b + c] = v;
-          part += (double)u[c] * (double)v;
-          break;
-        }
-        case TG_EP_RESID:
-          y[b + c] = f ? in.p[c] - au[c] : 0.0f;
-          break;
-        case TG_EP_RESID_NORM: {
-          const float v = f ? in.p[c] - au[c] : 0.0f;
-          y[b + c] = v;
-          part += (double)v * (double)v;
-          break;
-        }
-        case TG_EP_CHEB: {
-          const float t = in.q[c] * (in.p[c] - au[c]);
-          const float dn = f ? ((a2 != 0.0f) ? a1 * t + a2 * in.s[c] : a1 * t) : 0.0f;
-          d[b + c] = dn;
-          x_out[b + c] = f ? u[c] + dn : 0.0f;
-          break;
-        }
-        case TG_EP_CHEB_R: {
-          const float rn = in.p[c] - au[c];
-          const float dn = a1 * in.q[c] * rn + a2 * u[c];
-          r[b + c] = f ? rn : 0.0f;
-          d[b + c] = f ? dn : 0.0f;
-          x_out[b + c] = f ? in.s[c] + a3 * u[c] + dn : 0.0f;
-          break;
-        }
-        case TG_EP_RESID_UPDATE:
-          r[b + c] = f ? in.p[c] - au[c] : 0.0f;
-          break;
-        default:
-          break;
+      if constexpr (MODE == TG_EP_STORE) {
+        a.template st<EV_Y>(c, f ? au[c] : 0.0f);
+      } else if constexpr (MODE == TG_EP_DOT) {
+        const float v = f ? au[c] : 0.0f;
+        a.template st<EV_Y>(c, v);
+        part += (double)u[c] * (double)v;
+      } else if constexpr (MODE == TG_EP_RESID) {
+        a.template st<EV_Y>(c, f ? in.p[c] - au[c] : 0.0f);
+      } else if constexpr (MODE == TG_EP_RESID_NORM) {
+        const float v = f ? in.p[c] - au[c] : 0.0f;
+        a.template st<EV_Y>(c, v);
+        part += (double)v * (double)v;
+      } else if constexpr (MODE == TG_EP_CHEB) {
+        const float t = in.q[c] * (in.p[c] - au[c]);
+        const float dn = f ? ((e.a2 != 0.0f) ? e.a1 * t + e.a2 * in.s[c] : e.a1 * t) : 0.0f;
+        a.template st<EV_D>(c, dn);
+        a.template st<EV_X>(c, f ? u[c] + dn : 0.0f);
+      } else if constexpr (MODE == TG_EP_CHEB_R) {
+        const float rn = in.p[c] - au[c];
+        const float dn = e.a1 * in.q[c] * rn + e.a2 * u[c];
+        a.template st<EV_R>(c, f ? rn : 0.0f);
+        a.template st<EV_D>(c, f ? dn : 0.0f);
+        a.template st<EV_X>(c, f ? in.s[c] + e.a3 * u[c] + dn : 0.0f);
+      } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
+        a.template st<EV_R>(c, f ? in.p[c] - au[c] : 0.0f);
       }
     }
     return part;
   }
+};
 
+// Global-memory accessor bound to one node (b = 3 * node).
+struct PtrAcc {
+  const Ep32& e;
+  long long b;
+  template <int V>
+  __device__ __forceinline__ const float* ptr() const {
+    if constexpr (V == EV_RHS) return e.rhs;
+    else if constexpr (V == EV_DINV) return e.dinv;
+    else if constexpr (V == EV_D) return e.d;
+    else if constexpr (V == EV_X) return e.x_out;
+    else if constexpr (V == EV_R) return e.r;
+    else return e.y;
+  }
+  template <int V>
+  __device__ __forceinline__ float ld(int c) const { return ptr<V>()[b + c]; }
+  template <int V>
+  __device__ __forceinline__ void st(int c, float v) const { const_cast<float*>(ptr<V>())[b + c] = v; }
+};
+
+// Epilogue with the mode fixed at compile time, node-indexed interface.
+template <int MODE>
+struct Ep32M : Ep32 {
+  using Mode = EpMode<MODE>;
+  using In = typename Mode::In;
+  static constexpr int kMode = MODE;
+  __device__ __forceinline__ void load(long long node, In& in) const { Mode::load(*this, PtrAcc{*this, 3 * node}, in); }
+  __device__ __forceinline__ double store(long long node, unsigned m, const float (&au)[3], const float (&u)[3],
+                                          const In& in) const {
+    return Mode::store(*this, PtrAcc{*this, 3 * node}, m, au, u, in);
+  }
   __device__ __forceinline__ double operator()(long long node, unsigned m, const float (&au)[3],
                                                const float (&u)[3]) const {
     In in;
     load(node, in);
     return store(node, m, au, u, in);
   }
-  __host__ __device__ bool reduces() const { return mode == TG_EP_DOT || mode == TG_EP_RESID_NORM; }
+  __host__ __device__ static constexpr bool reduces() { return Mode::kReduces; }
 };
+
+// Call f(Ep32M<mode>{e}) for the runtime mode of e; returns f's status, or
+// TG_ERR_ARG for an unknown mode.
+template <class F>
+int with_mode(const Ep32& e, F&& f) {
+  switch (e.mode) {
+    case TG_EP_STORE: return f(Ep32M<TG_EP_STORE>{e});
+    case TG_EP_DOT: return f(Ep32M<TG_EP_DOT>{e});
+    case TG_EP_RESID: return f(Ep32M<TG_EP_RESID>{e});
+    case TG_EP_RESID_NORM: return f(Ep32M<TG_EP_RESID_NORM>{e});
+    case TG_EP_CHEB: return f(Ep32M<TG_EP_CHEB>{e});
+    case TG_EP_CHEB_R: return f(Ep32M<TG_EP_CHEB_R>{e});
+    case TG_EP_RESID_UPDATE: return f(Ep32M<TG_EP_RESID_UPDATE>{e});
+    default: return set_error(TG_ERR_ARG, "unknown epilogue mode %d", e.mode);
+  }
+}
 
 // FP64 level-0 operator output: y = A u or y = rhs - A u, optional FP32 copy
 // and sum of squares.

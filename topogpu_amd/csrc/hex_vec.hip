@@ -10,6 +10,8 @@
 // A workgroup stages the node brick of its 4x4x16 node tile plus a one-node
 // halo and the element weights in LDS; each thread then holds its 27-node
 // neighbourhood in registers.
+#include <type_traits>
+
 #include "common.h"
 #include "epilogue.h"
 #include "reduce.h"
@@ -302,7 +304,9 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
   switch (op->kind) {
     case TG_OP_HEX_F32:
       TG_REQUIRE(op->weights, "weights NULL");
-      return launch_node<float>(g, op->mask, HexW<float>{op->weights}, u, e, ep->dot, s);
+      return with_mode(e, [&](const auto& em) {
+        return launch_node<float>(g, op->mask, HexW<float>{op->weights}, u, em, ep->dot, s);
+      });
     case TG_OP_HEX_BF16:
     case TG_OP_HEX_BF16X3:
       TG_REQUIRE(op->weights, "weights NULL");
@@ -310,15 +314,20 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
     case TG_OP_Q8_F32: {
       TG_REQUIRE(op->weights, "weights NULL");
       const Grid fine{2 * g.nx, 2 * g.ny, 2 * g.nz};
-      return launch_node<float>(g, op->mask, Q8W{op->weights, fine}, u, e, ep->dot, s);
+      return with_mode(e, [&](const auto& em) {
+        return launch_node<float>(g, op->mask, Q8W{op->weights, fine}, u, em, ep->dot, s);
+      });
     }
     case TG_OP_STORED_F32: {
       TG_REQUIRE(op->elem_mats, "elem_mats NULL");
       const int nb = node_tiles(g);
-      hipLaunchKernelGGL((k_node_stored<Ep32>), dim3(nb), dim3(NT), 0, s, g, op->mask, op->elem_mats, u, e,
-                         ep->dot, workspace());
-      TG_LAUNCH_CHECK();
-      return e.reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
+      return with_mode(e, [&](const auto& em) {
+        using E = std::decay_t<decltype(em)>;
+        hipLaunchKernelGGL((k_node_stored<E>), dim3(nb), dim3(NT), 0, s, g, op->mask, op->elem_mats, u, em,
+                           ep->dot, workspace());
+        TG_LAUNCH_CHECK();
+        return E::reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
+      });
     }
     default:
       return set_error(TG_ERR_ARG, "unknown operator kind %d", op->kind);

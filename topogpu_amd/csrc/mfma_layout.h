@@ -19,4 +19,15 @@ __host__ __device__ constexpr int mfma_b_row(int lane, int v) { return 8 * (lane
 __host__ __device__ constexpr int mfma_d_row(int tile, int lane, int v) { return 16 * tile + 4 * (lane >> 4) + v; }
 __host__ __device__ constexpr int mfma_d_col(int lane) { return lane & 15; }
 
+// Output-row order of the element operator in the MFMA kernel. D tile t holds
+// the z-face t of the element (t = corner z offset); within a tile, row group
+// g = (row % 16) / 4 holds one corner with x offset g & 1. The y offset of the
+// group alternates between two layouts so that the y-line shared by
+// neighbouring pencils py (layout py & 1) and py + 1 lands in the same lane
+// half of both:
+//   layout 0: groups 0,1 -> y offset 1, groups 2,3 -> y offset 0
+//   layout 1: groups 0,1 -> y offset 0, groups 2,3 -> y offset 1
+__host__ __device__ constexpr int mfma_out_cx(int g) { return g & 1; }
+__host__ __device__ constexpr int mfma_out_cy(int layout, int g) { return (g >> 1) ^ (1 - layout); }
+
 }  // namespace tg
