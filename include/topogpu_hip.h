@@ -247,15 +247,21 @@ int tg_sensitivity64(const tg_grid* g, const double* U, const double* rho_phys,
 
 /* OC bisection of pub_simp_solver.py:319-340 on the device.
  * oc_prepare: q = rho*sqrt(-min(dc,-1e-12)/max(dv,1e-12)),
- *   lo = max(rho-move, 1e-3), hi = min(rho+move, 1) ; state <- {l1=0, l2=1e9}.
- * oc_bisect_step: advances the bisection by up to 4 steps (no-op once
- *   converged). Each step: lm=(l1+l2)/2, rn = clip(q/sqrt(lm), lo, hi),
- *   volume = sum(heaviside(H rn))/n, update the bracket; converged when
- *   l2-l1 < 1e-9 or after 100 steps. The 15 midpoints of the next 4 levels
- *   are evaluated in one pass; the volume is monotone in lambda, so the
- *   decisions equal those of 4 sequential steps. Filter as in tg_stencil64.
+ *   lo = rho-move, hi = rho+move ; state <- {l1=0, l2=1e9} and the first
+ *   batch of trial multipliers.
+ * oc_bisect_step: one evaluation pass: the filtered+projected volume
+ *   sum(heaviside(H clip(q/sqrt(lambda), lo, hi, 1e-3, 1))) at 15 trial
+ *   multipliers, then the bisection (lm=(l1+l2)/2, l1=lm if volume/n >
+ *   volfrac else l2=lm, stop when l2-l1 < 1e-9 or after 100 steps) advances
+ *   through every step these volumes decide -- at least two per pass, since
+ *   the next two levels of midpoints are always among the trial multipliers;
+ *   the others bracket the volume crossing, which decides whole runs of
+ *   later midpoints by monotonicity. The decisions are those of the
+ *   sequential loop. No-op once converged. Filter as in tg_stencil64.
  * oc_finalize: rho_new = clip(q/sqrt(lm_last)); *change = max|rho_new-rho|.
- * state: 8 device doubles {l1, l2, lm, done, steps, vol, -, -}. */
+ * state: TG_OC_STATE_LEN device doubles {l1, l2, lm_last, done, steps,
+ *   passes, ...}. */
+#define TG_OC_STATE_LEN 32
 int tg_oc_prepare(int64_t n, const double* rho, const double* dc, const double* dv, double move,
                   double* q, double* lo, double* hi, double* state, void* stream);
 int tg_oc_bisect_step(const tg_grid* g, int32_t m, const int32_t* offs_host,

@@ -198,18 +198,21 @@ def matfree_diag(E_e: np.ndarray, edof: np.ndarray, free_mask: np.ndarray, ndof:
     return np.where(free_mask, d, 1.0)
 
 
-def oc_update_filtered(rho, dc, dv, volfrac, move, H, beta, eta, use_heaviside):
+def oc_update_filtered(rho, dc, dv, volfrac, move, H, beta, eta, use_heaviside, return_state=False):
     """OC with filtered+projected volume constraint (pub_simp_solver.py:319-340).
 
     Bisection on lambda in [0, 1e9], at most 100 steps, stop when the bracket
-    is narrower than 1e-9; returns the candidate of the last midpoint.
+    is narrower than 1e-9; returns the candidate of the last midpoint (and,
+    with ``return_state``, also the last midpoint and the number of steps).
     """
     n = rho.shape[0]
     dc_s = np.minimum(dc, -1e-12)
     dv_s = np.maximum(dv, 1e-12)
     l1, l2 = 0.0, 1e9
     rn = rho
+    lm, steps = 0.5 * (l1 + l2), 0
     for _ in range(100):
+        steps += 1
         lm = 0.5 * (l1 + l2)
         rn = rho * np.sqrt(-dc_s / (lm * dv_s))
         rn = np.clip(rn, rho - move, rho + move)
@@ -222,7 +225,7 @@ def oc_update_filtered(rho, dc, dv, volfrac, move, H, beta, eta, use_heaviside):
             l2 = lm
         if l2 - l1 < 1e-9:
             break
-    return rn
+    return (rn, lm, steps) if return_state else rn
 
 
 def oc_update_mean(rho, dc, volfrac, move, rho_min):

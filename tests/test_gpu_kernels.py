@@ -365,15 +365,25 @@ def test_density_chain_and_sensitivity():
     assert rel_inf(h(dcf), -3.0 * (1 - 1e-9) * rp ** 2 * ce_ref * dh) < 1e-13
 
 
-def test_oc_bisection_matches_oracle():
-    dims = (10, 6, 5)
+@pytest.mark.parametrize("dims,seed,volfrac,spread", [
+    ((10, 6, 5), 12, 0.45, 0.0),    # crossing inside the bracket
+    ((10, 6, 5), 12, 0.30, 0.0),    # volume above target even at lambda = 1e9: all steps go right
+    ((24, 12, 6), 3, 0.30, 6.0),    # sensitivities over six decades, crossing near 35
+    ((24, 12, 6), 3, 0.45, 6.0),
+    ((30, 20, 10), 5, 0.45, 6.0),   # crossing near 5e-5
+])
+def test_oc_bisection_matches_oracle(dims, seed, volfrac, spread):
+    """The planned passes reach the sequential bisection's last midpoint and
+    step count (pub_simp_solver.py:319-340 via the oracle), and the same
+    updated densities."""
     n = int(np.prod(dims))
-    rng = np.random.default_rng(12)
+    rng = np.random.default_rng(seed)
     rho = rng.uniform(0.05, 0.95, n)
-    dc = -rng.uniform(0.01, 5.0, n)
+    dc = -rng.uniform(0.01, 5.0, n) * 10.0 ** rng.uniform(-spread, 0.0, n)
     dv = rng.uniform(0.5, 1.5, n)
     H = oracle.build_density_filter(*dims, 1.5)
-    ref = oracle.oc_update_filtered(rho, dc, dv, 0.45, 0.2, H, 1.0, 0.5, True)
+    ref, lm_ref, steps_ref = oracle.oc_update_filtered(rho, dc, dv, volfrac, 0.2, H, 1.0, 0.5, True,
+                                                       return_state=True)
     offs, w = fem.filter_stencil(1.5)
     # Hs = W 1 from the raw stencil on the host
     Wsum = np.zeros(dims)
@@ -383,21 +393,25 @@ def test_oc_bisection_matches_oracle():
         Wsum += wk * c
     hs_inv = 1.0 / Wsum.ravel()
     q, lo, hi, rn = (torch.zeros(n, dtype=torch.float64, device=DEV) for _ in range(4))
-    state = torch.zeros(8, dtype=torch.float64, device=DEV)
+    state = torch.zeros(L.OC_STATE_LEN, dtype=torch.float64, device=DEV)
     chg = torch.zeros(1, dtype=torch.float64, device=DEV)
     drho = t(rho)
     L.call("tg_oc_prepare", n, L.ptr(drho), L.ptr(t(dc)), L.ptr(t(dv)), 0.2, L.ptr(q), L.ptr(lo), L.ptr(hi),
            L.ptr(state), stream())
     dhs = t(hs_inv)
-    for _ in range(100):
+    for _ in range(50):   # calls after convergence are no-ops
         L.call("tg_oc_bisect_step", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(dhs),
-               L.ptr(q), L.ptr(lo), L.ptr(hi), 1.0, 0.5, 0.45, 1, L.ptr(state), stream())
+               L.ptr(q), L.ptr(lo), L.ptr(hi), 1.0, 0.5, volfrac, 1, L.ptr(state), stream())
     L.call("tg_oc_finalize", n, L.ptr(drho), L.ptr(q), L.ptr(lo), L.ptr(hi), L.ptr(state), L.ptr(rn), L.ptr(chg),
            stream())
+    st = h(state)
+    assert st[3] == 1.0
+    assert int(st[4]) == steps_ref
+    assert st[2] == lm_ref
+    assert st[5] <= 8          # evaluation passes (the tree method needs steps/4)
     got = h(rn)
-    assert np.abs(got - ref).max() < 1e-9
-    assert h(chg)[0] == pytest.approx(np.abs(ref - rho).max(), abs=1e-9)
-    assert 55 <= h(state)[4] <= 100
+    assert np.abs(got - ref).max() < 1e-12
+    assert h(chg)[0] == pytest.approx(np.abs(ref - rho).max(), abs=1e-12)
 
 
 def test_krylov_vector_kernels():

@@ -113,17 +113,31 @@ __device__ __forceinline__ double oc_candidate(double q, double lo, double hi, d
   return fmin(fmax(fmin(fmax(x, lo), hi), 1e-3), 1.0);
 }
 
+// OC bisection state (device doubles).
+enum OcState {
+  S_L1 = 0, S_L2, S_LM, S_DONE, S_STEPS, S_PASSES,
+  S_A, S_VA,       // largest multiplier known to give volume above the target, and that volume
+  S_B, S_VB,       // smallest multiplier known to give volume at or below the target
+  S_SA, S_SB,      // as S_A / S_B, restricted to volumes clear of the target by the margin
+  S_CAND,          // OC_NPTS multipliers of the next evaluation pass
+  S_LEN = TG_OC_STATE_LEN
+};
+constexpr int OC_NPTS = 15;
+static_assert(S_CAND + OC_NPTS <= S_LEN, "state layout");
+
 __global__ void k_oc_prepare(long long n, const double* __restrict__ rho, const double* __restrict__ dc,
                              const double* __restrict__ dv, double move, double* q, double* lo, double* hi,
                              double* state) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e == 0) {
-    state[0] = 0.0;   // l1
-    state[1] = 1e9;   // l2
-    state[2] = 0.5e9; // lm
-    state[3] = 0.0;   // done
-    state[4] = 0.0;   // steps
-    state[5] = 0.0;   // volume of last step
+    for (int k = 0; k < S_LEN; ++k) state[k] = 0.0;
+    state[S_L1] = 0.0;
+    state[S_L2] = 1e9;
+    state[S_LM] = 0.5e9;
+    state[S_A] = state[S_SA] = -1.0;      // none yet
+    state[S_B] = state[S_SB] = HUGE_VAL;  // none yet
+    // first pass: multipliers log-spaced over [1e-10, 1e9] (ratio 10^(19/14))
+    for (int t = 0; t < OC_NPTS; ++t) state[S_CAND + t] = pow(10.0, -10.0 + 19.0 * t / (OC_NPTS - 1));
   }
   if (e >= n) return;
   const double dcs = fmin(dc[e], -1e-12), dvs = fmax(dv[e], 1e-12);
@@ -132,40 +146,18 @@ __global__ void k_oc_prepare(long long n, const double* __restrict__ rho, const 
   hi[e] = rho[e] + move;
 }
 
-// One OC pass covers OC_LEVELS bisection steps at once. The 2^L - 1 candidate
-// multipliers are the midpoints of the binary tree below the current bracket
-// [l1, l2], each computed as 0.5*(a+b) of its own bracket -- the same
-// arithmetic the sequential bisection performs on whichever path it takes.
-// The filtered/projected volume is monotone in lambda, so walking the tree
-// with the evaluated volumes reproduces the sequential decisions exactly.
-constexpr int OC_LEVELS = 4;
-constexpr int OC_NPTS = (1 << OC_LEVELS) - 1;   // tree nodes in heap order
-
-__device__ __forceinline__ void oc_tree(double l1, double l2, double (&lam)[OC_NPTS]) {
-  double lo_[OC_NPTS], hi_[OC_NPTS];
-  lo_[0] = l1;
-  hi_[0] = l2;
-#pragma unroll
-  for (int t = 0; t < OC_NPTS; ++t) {
-    lam[t] = 0.5 * (lo_[t] + hi_[t]);
-    if (2 * t + 2 < OC_NPTS) {
-      lo_[2 * t + 1] = lo_[t];  hi_[2 * t + 1] = lam[t];   // volume too small -> left
-      lo_[2 * t + 2] = lam[t];  hi_[2 * t + 2] = hi_[t];   // volume too large -> right
-    }
-  }
-}
-
+// Volume sum(heaviside(H x(lambda))) at the OC_NPTS multipliers of the state,
+// one partial per block and multiplier.
 __global__ void __launch_bounds__(256) k_oc_pass(Grid g, Stencil st, const double* __restrict__ hs_inv,
                                                 const double* __restrict__ q, const double* __restrict__ lo,
                                                 const double* __restrict__ hi, double beta, double eta,
                                                 int use_h, const double* state, Workspace ws) {
-  if (state[3] != 0.0) return;   // uniform: converged bracket
-  double lam[OC_NPTS], rs[OC_NPTS];
-  oc_tree(state[0], state[1], lam);
+  if (state[S_DONE] != 0.0) return;   // uniform: converged bracket
   // 1/sqrt(lambda) once per candidate: the element loop then multiplies
   // (an FP64 divide per neighbour and candidate would dominate the pass)
+  double rs[OC_NPTS];
 #pragma unroll
-  for (int t = 0; t < OC_NPTS; ++t) rs[t] = 1.0 / sqrt(lam[t]);
+  for (int t = 0; t < OC_NPTS; ++t) rs[t] = 1.0 / sqrt(state[S_CAND + t]);
   double v[OC_NPTS];
 #pragma unroll
   for (int t = 0; t < OC_NPTS; ++t) v[t] = 0.0;
@@ -195,11 +187,122 @@ __global__ void __launch_bounds__(256) k_oc_pass(Grid g, Stencil st, const doubl
   block_partials<OC_NPTS>(v, ws);
 }
 
-// One block: the volumes of the pass's candidates, then up to OC_LEVELS
-// bracket updates of pub_simp_solver.py:333-338, stopping when l2-l1 < 1e-9.
-__global__ void __launch_bounds__(256) k_oc_walk(int nb, long long n, double volfrac, int max_steps, double* state,
+// Tiled form of k_oc_pass for stencils of radius <= OC_R. A block owns an
+// OTX x OTY x OTZ element tile; for each group of OC_G multipliers it writes
+// the clipped candidate densities of the tile and its halo to LDS once (one
+// clip per element and multiplier instead of one per stencil neighbour),
+// then every thread filters and projects a z-run of OC_RUN elements from LDS.
+constexpr int OTX = 8, OTY = 8, OTZ = 16, OC_THREADS = 256, OC_RUN = OTX * OTY * OTZ / OC_THREADS;
+constexpr int OC_R = 1, OC_G = 3;
+static_assert(OC_NPTS % OC_G == 0, "multiplier groups");
+static_assert(OTZ % OC_RUN == 0, "z-runs tile the z extent");
+
+__global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st, int R, const double* __restrict__ hs_inv,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ lo,
+                                                             const double* __restrict__ hi, double beta, double eta,
+                                                             int use_h, const double* state, Workspace ws) {
+  if (state[S_DONE] != 0.0) return;   // uniform: converged bracket
+  constexpr int HMAX = (OTX + 2 * OC_R) * (OTY + 2 * OC_R) * (OTZ + 2 * OC_R);
+  constexpr int HIT = (HMAX + OC_THREADS - 1) / OC_THREADS;
+  __shared__ double sx[OC_G][HMAX];
+  const int HX = OTX + 2 * R, HY = OTY + 2 * R, HZ = OTZ + 2 * R, NH = HX * HY * HZ;
+  const int ntz = (g.nz + OTZ - 1) / OTZ, nty = (g.ny + OTY - 1) / OTY;
+  const int bt = blockIdx.x;
+  const int k0 = (bt % ntz) * OTZ, j0 = ((bt / ntz) % nty) * OTY, i0 = (bt / (ntz * nty)) * OTX;
+
+  const bool proj_on = use_h && beta > 1e-6;
+  const double p0 = tanh(beta * eta), pden = p0 + tanh(beta * (1.0 - eta));
+  // this thread's z-run
+  const int tz = threadIdx.x % (OTZ / OC_RUN), ty = (threadIdx.x / (OTZ / OC_RUN)) % OTY,
+            tx = threadIdx.x / ((OTZ / OC_RUN) * OTY);
+  const int oi = i0 + tx, oj = j0 + ty, ok0 = k0 + tz * OC_RUN;
+  double hinv[OC_RUN];
+  bool own[OC_RUN];
+#pragma unroll
+  for (int r = 0; r < OC_RUN; ++r) {
+    own[r] = oi < g.nx && oj < g.ny && ok0 + r < g.nz;
+    hinv[r] = own[r] ? hs_inv[g.elem(oi, oj, ok0 + r)] : 0.0;
+  }
+  const int obase = ((tx + R) * HY + (ty + R)) * HZ + tz * OC_RUN + R;
+
+  const int nb = gridDim.x;
+#pragma unroll 1
+  for (int grp = 0; grp < OC_NPTS / OC_G; ++grp) {
+    if (grp > 0) __syncthreads();   // previous group's reads are done
+    double rs[OC_G];
+#pragma unroll
+    for (int c = 0; c < OC_G; ++c) rs[c] = 1.0 / sqrt(state[S_CAND + grp * OC_G + c]);
+    // halo inputs (re-read per group from L2; out-of-grid: no density)
+#pragma unroll
+    for (int it = 0; it < HIT; ++it) {
+      const int idx = threadIdx.x + it * OC_THREADS;
+      const int hz = idx % HZ, hy = (idx / HZ) % HY, hx = idx / (HZ * HY);
+      const int i = i0 - R + hx, j = j0 - R + hy, k = k0 - R + hz;
+      const bool in = idx < NH && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      const long long f = in ? g.elem(i, j, k) : 0;
+      const double hq = in ? q[f] : 0.0, hl = in ? lo[f] : 0.0, hh = in ? hi[f] : 0.0;
+      if (idx < NH)
+#pragma unroll
+        for (int c = 0; c < OC_G; ++c)
+          sx[c][idx] = in ? fmin(fmax(fmin(fmax(hq * rs[c], hl), hh), 1e-3), 1.0) : 0.0;
+    }
+    __syncthreads();
+    double acc[OC_G][OC_RUN];
+#pragma unroll
+    for (int c = 0; c < OC_G; ++c)
+#pragma unroll
+      for (int r = 0; r < OC_RUN; ++r) acc[c][r] = 0.0;
+    for (int o = 0; o < st.m; ++o) {
+      const int off = obase + (st.off[o][0] * HY + st.off[o][1]) * HZ + st.off[o][2];
+      const double w = st.w[o];
+#pragma unroll
+      for (int c = 0; c < OC_G; ++c)
+#pragma unroll
+        for (int r = 0; r < OC_RUN; ++r) acc[c][r] = fma(w, sx[c][off + r], acc[c][r]);
+    }
+    double vg[OC_G];
+#pragma unroll
+    for (int c = 0; c < OC_G; ++c) {
+      vg[c] = 0.0;
+#pragma unroll
+      for (int r = 0; r < OC_RUN; ++r)
+        if (own[r]) {
+          const double rf = acc[c][r] * hinv[r];
+          vg[c] += proj_on ? (p0 + tanh(beta * (rf - eta))) / pden : rf;
+        }
+    }
+    // partials in the layout of block_partials<OC_NPTS>
+    block_sum<OC_G>(vg);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < OC_G; ++c) ws.partials[(size_t)(grp * OC_G + c) * nb + blockIdx.x] = vg[c];
+  }
+}
+
+// One block: sums the pass's volumes, then
+//  1. tightens the bracket [a, b] of the volume crossing;
+//  2. advances the bisection of pub_simp_solver.py:333-338 (lm = (l1+l2)/2,
+//     l1 = lm when volume/n > volfrac else l2 = lm, stop when l2-l1 < 1e-9
+//     or after max_steps) through every step the evaluations decide: a
+//     midpoint that was itself evaluated uses its own volume, exactly like
+//     the sequential loop; a midpoint at or below an evaluated multiplier
+//     whose volume is above the target (at or above one whose volume is at
+//     or below it) takes that decision, which the monotonicity of the volume
+//     in lambda implies. Only evaluations whose volume clears the target by
+//     more than any rounding-level non-monotonicity could bridge are used
+//     this way (sa, sb);
+//  3. chooses the next pass's multipliers: the three midpoints of the next
+//     two bisection levels (exact decisions, so every pass advances at least
+//     two steps), one third-level midpoint, and a secant prediction of the
+//     crossing in s = lambda^(-1/2) (the unclipped volume is linear in s)
+//     with ten geometrically spaced neighbours, which shrink [a, b] around
+//     the crossing so that whole runs of later midpoints are decided. Once
+//     [a, b] is narrower than the bisection's final resolution, the pass
+//     evaluates the 15 midpoints of the next four levels instead.
+__global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double volfrac, int max_steps, double* state,
                                                 Workspace ws) {
-  if (state[3] != 0.0) return;
+  if (state[S_DONE] != 0.0) return;
   __shared__ double svol[OC_NPTS];
   for (int t = 0; t < OC_NPTS; ++t) {
     double v[1] = {0.0};
@@ -207,29 +310,94 @@ __global__ void __launch_bounds__(256) k_oc_walk(int nb, long long n, double vol
     block_sum<1>(v);
     if (threadIdx.x == 0) svol[t] = v[0];
   }
-  if (threadIdx.x == 0) {
-    double lam[OC_NPTS];
-    oc_tree(state[0], state[1], lam);
-    int node = 0;
-    for (int lev = 0; lev < OC_LEVELS; ++lev) {
-      if (state[4] >= max_steps) { state[3] = 1.0; break; }
-      const double lm = lam[node];
-      state[5] = svol[node];
-      const bool right = svol[node] / (double)n > volfrac;
-      if (right) state[0] = lm; else state[1] = lm;
-      state[2] = lm;
-      state[4] += 1.0;
-      if (state[1] - state[0] < 1e-9 || state[4] >= max_steps) { state[3] = 1.0; break; }
-      node = 2 * node + (right ? 2 : 1);
+  if (threadIdx.x != 0) return;
+  const double dn = (double)n;
+  constexpr double kMargin = 2e-14;   // on volume/n; a faithfully rounded projection moves it by ~1e-16
+  double cand[OC_NPTS];
+  for (int t = 0; t < OC_NPTS; ++t) cand[t] = state[S_CAND + t];
+  double a = state[S_A], va = state[S_VA], b = state[S_B], vb = state[S_VB];
+  double sa = state[S_SA], sb = state[S_SB];
+  for (int t = 0; t < OC_NPTS; ++t) {
+    const double f = svol[t] / dn;
+    if (f > volfrac) {
+      if (cand[t] > a) { a = cand[t]; va = svol[t]; }
+      if (f - volfrac > kMargin && cand[t] > sa) sa = cand[t];
+    } else {
+      if (cand[t] < b) { b = cand[t]; vb = svol[t]; }
+      if (volfrac - f > kMargin && cand[t] < sb) sb = cand[t];
     }
   }
+  double l1 = state[S_L1], l2 = state[S_L2], lm_last = state[S_LM], steps = state[S_STEPS];
+  bool done = false;
+  while (!done) {
+    const double lm = 0.5 * (l1 + l2);
+    int dec = -1;   // 1: volume above target (l1 = lm), 0: at or below (l2 = lm)
+    for (int t = 0; t < OC_NPTS && dec < 0; ++t)
+      if (cand[t] == lm) dec = svol[t] / dn > volfrac ? 1 : 0;
+    if (dec < 0 && sa >= 0.0 && lm <= sa) dec = 1;
+    if (dec < 0 && sb < HUGE_VAL && lm >= sb) dec = 0;
+    if (dec < 0) break;
+    if (dec) l1 = lm; else l2 = lm;
+    lm_last = lm;
+    steps += 1.0;
+    if (l2 - l1 < 1e-9 || steps >= max_steps) done = true;
+  }
+  state[S_L1] = l1; state[S_L2] = l2; state[S_LM] = lm_last; state[S_STEPS] = steps;
+  state[S_A] = a; state[S_VA] = va; state[S_B] = b; state[S_VB] = vb;
+  state[S_SA] = sa; state[S_SB] = sb;
+  state[S_PASSES] += 1.0;
+  state[S_DONE] = done ? 1.0 : 0.0;
+  if (done) return;
+
+  // next pass
+  double nc[OC_NPTS];
+  if (a >= 0.0 && b < HUGE_VAL && b - a < 1e-9) {
+    // crossing resolved beyond the final bracket width: the next four
+    // levels of midpoints, in heap order
+    double lo_[OC_NPTS], hi_[OC_NPTS];
+    lo_[0] = l1;
+    hi_[0] = l2;
+    for (int t = 0; t < OC_NPTS; ++t) {
+      nc[t] = 0.5 * (lo_[t] + hi_[t]);
+      if (2 * t + 2 < OC_NPTS) {
+        lo_[2 * t + 1] = lo_[t]; hi_[2 * t + 1] = nc[t];
+        lo_[2 * t + 2] = nc[t];  hi_[2 * t + 2] = hi_[t];
+      }
+    }
+    for (int t = 0; t < OC_NPTS; ++t) state[S_CAND + t] = nc[t];
+    return;
+  }
+  const double m0 = 0.5 * (l1 + l2), mlo = 0.5 * (l1 + m0), mhi = 0.5 * (m0 + l2);
+  nc[0] = m0; nc[1] = mlo; nc[2] = mhi;
+  const double hb = fmin(b, l2);   // l2 >= b whenever b exists
+  double P = m0;                   // without a two-sided bracket: follow the bisection
+  double L = 1.0;                  // log-width of the bracket
+  if (a > 0.0 && b < HUGE_VAL && a < hb) {
+    const double sa = 1.0 / sqrt(a), sb = 1.0 / sqrt(b);
+    double s = 0.5 * (sa + sb);
+    if (va > vb) s = sb + (volfrac * dn - vb) * (sa - sb) / (va - vb);
+    P = 1.0 / (s * s);
+    if (!(P > a && P < b)) P = sqrt(a * b);
+    L = log(b / a);
+  }
+  // third-level midpoint on the side the prediction points to
+  nc[3] = (P > m0) ? ((P > mhi) ? 0.5 * (mhi + l2) : 0.5 * (m0 + mhi))
+                   : ((P > mlo) ? 0.5 * (mlo + m0) : 0.5 * (l1 + mlo));
+  nc[4] = P;
+  double dlt = (L > 0.0 && L < 50.0) ? L : 1.0;
+  for (int k = 0; k < 5; ++k) {
+    dlt *= 0.1;
+    nc[5 + 2 * k] = P * exp(-dlt);
+    nc[6 + 2 * k] = P * exp(dlt);
+  }
+  for (int t = 0; t < OC_NPTS; ++t) state[S_CAND + t] = nc[t] > 0.0 ? nc[t] : m0;
 }
 
 __global__ void __launch_bounds__(256) k_oc_finalize(long long n, const double* __restrict__ rho,
                                                     const double* __restrict__ q, const double* __restrict__ lo,
                                                     const double* __restrict__ hi, const double* state,
                                                     double* rho_new, Workspace ws) {
-  const double lm = state[2];
+  const double lm = state[S_LM];
   double mx = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
     const double r = oc_candidate(q[e], lo[e], hi[e], lm);
@@ -326,11 +494,23 @@ extern "C" int tg_oc_bisect_step(const tg_grid* gp, int32_t m, const int32_t* of
   Stencil st;
   if (int r = make_stencil(m, offs, w, st)) return r;
   const Grid g = to_grid(gp);
-  const int nb = grid_blocks(g.elems());
-  hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(256), 0, (hipStream_t)s, g, st, hs_inv, q, lo, hi, beta, eta, use_h,
-                     state, workspace());
+  int R = 0;
+  for (int k = 0; k < st.m; ++k)
+    for (int d = 0; d < 3; ++d) R = st.off[k][d] > R ? st.off[k][d] : (-st.off[k][d] > R ? -st.off[k][d] : R);
+  int nb;
+  if (R <= OC_R) {
+    const long long tiles = (long long)((g.nx + OTX - 1) / OTX) * ((g.ny + OTY - 1) / OTY) * ((g.nz + OTZ - 1) / OTZ);
+    TG_REQUIRE(tiles * OC_NPTS <= kMaxPartials, "grid too large for the reduction workspace");
+    nb = (int)tiles;
+    hipLaunchKernelGGL(k_oc_pass_tiled, dim3(nb), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R, hs_inv, q, lo, hi,
+                       beta, eta, use_h, state, workspace());
+  } else {
+    nb = grid_blocks(g.elems());
+    hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(256), 0, (hipStream_t)s, g, st, hs_inv, q, lo, hi, beta, eta,
+                       use_h, state, workspace());
+  }
   TG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_oc_walk, dim3(1), dim3(256), 0, (hipStream_t)s, nb, g.elems(), volfrac, 100, state,
+  hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(256), 0, (hipStream_t)s, nb, g.elems(), volfrac, 100, state,
                      workspace());
   TG_LAUNCH_CHECK();
   return TG_OK;

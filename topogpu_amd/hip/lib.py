@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent.parent / "libtopohip.so"
 
 TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32 = 1, 2, 3, 4, 5
 TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE = range(7)
+OC_STATE_LEN = 32   # device doubles of the OC bisection state (TG_OC_STATE_LEN)
 
 _vp = C.c_void_p
 _i32 = C.c_int32

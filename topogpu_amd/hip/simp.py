@@ -62,7 +62,7 @@ class HipSIMP:
         self.q = torch.zeros(self.n_elem, **f64)
         self.lo = torch.zeros(self.n_elem, **f64)
         self.hi = torch.zeros(self.n_elem, **f64)
-        self.oc_state = torch.zeros(8, **f64)
+        self.oc_state = torch.zeros(L.OC_STATE_LEN, **f64)
         self.scal = torch.zeros(4, **f64)
 
     def _row_sums(self) -> np.ndarray:
@@ -117,19 +117,18 @@ class HipSIMP:
         L.call("tg_oc_prepare", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), L.ptr(self.dv), self.move,
                L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), L.ptr(self.oc_state), st)
         use_h = int(self.use_heaviside and self.beta > 1e-6)
-        done = 0
-        steps = 0
-        # each call advances up to 4 bisection steps; a [0, 1e9] bracket
-        # needs 60 steps to shrink below 1e-9, i.e. 15 calls
-        while steps < max_steps:
-            chunk = 15 if steps == 0 else 2
+        # each call evaluates one batch of multipliers and advances the
+        # bisection by at least two steps (usually far more); calls after
+        # convergence return at once, so a few are queued between checks
+        passes = 0
+        while passes < (max_steps + 1) // 2:
+            chunk = 5 if passes == 0 else 2
             for _ in range(chunk):
                 L.call("tg_oc_bisect_step", self.grid, self.m, self.offs.ctypes.data, self.w.ctypes.data,
                        L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
                        self.volfrac, use_h, L.ptr(self.oc_state), st)
-            steps += 4 * chunk
-            done = float(self.oc_state[3].item())
-            if done:
+            passes += chunk
+            if float(self.oc_state[3].item()):
                 break
         L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
