@@ -66,7 +66,11 @@ enum {
    * matrix sum_s E_child(s) * Q_s, weights = fine E (FP32)                 */
   TG_OP_Q8_F32 = 4,
   /* stored element matrices (576 FP32 per element, row-major)               */
-  TG_OP_STORED_F32 = 5
+  TG_OP_STORED_F32 = 5,
+  /* assembled 27-point block stencil from tg_assemble_stencil (243 FP32 per
+   * node); input entries at fixed DOFs must be finite (their coefficients
+   * are zero)                                                              */
+  TG_OP_STENCIL_F32 = 6
 };
 
 typedef struct {
@@ -75,6 +79,7 @@ typedef struct {
   const uint8_t* mask;        /* per-node free mask of that grid             */
   const float* weights;       /* HEX*: E[elem]; Q8: fine E[fine elem]        */
   const float* elem_mats;     /* STORED: 576 per element                     */
+  const float* stencil;       /* STENCIL: coefficients, see tg_assemble_stencil */
 } tg_op32;
 
 enum {
@@ -100,6 +105,16 @@ typedef struct {
   float a1, a2, a3;     /* CHEB / CHEB_R coefficients                       */
   double* dot;          /* device scalar for DOT / RESID_NORM               */
 } tg_ep32;
+
+/* Coarse-level operator M K M as a 27-point block stencil: for node n, row
+ * component c, neighbour q = (dx+1)*9+(dy+1)*3+(dz+1) and column component
+ * d, coef[(c*81 + 3*q + d) * nodes + n] = sum over the elements containing
+ * n and its neighbour of the element matrix entry, zero for fixed rows,
+ * fixed columns and neighbours outside the grid. mats: 576 FP64 per element
+ * (row-major, reference local node order). Replaces the coarse CSR matrices
+ * K_l = R K_{l-1} P of multigrid_v4.py:747-752 on the V-cycle path. */
+int tg_assemble_stencil(const tg_grid* g, const uint8_t* mask, const double* mats, float* coef,
+                        void* stream);
 
 /* y (or the epilogue outputs) = A u for a single-precision operator.
  * Replaces FusedMatvec.matvec / _launch (cuda_fused_matvec.py:336-409),

@@ -227,6 +227,57 @@ def _apply_stored(dims, mask, mats32, u):
     return h(y)
 
 
+def _stencil_op(dims, mask, m64):
+    nodes = int(np.prod([d + 1 for d in dims]))
+    coef = torch.full((243 * nodes,), np.nan, dtype=torch.float32, device=DEV)
+    dm = t(mask)
+    L.call("tg_assemble_stencil", L.grid(dims), L.ptr(dm), L.ptr(m64), L.ptr(coef), stream())
+    # the caller holds coef and dm for as long as it uses the operator
+    return L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dims), L.ptr(dm), None, None, L.ptr(coef)), coef, dm
+
+
+def _check_stencil_epilogues(op, K, fr, nd, rng):
+    """Every epilogue of the assembled-stencil operator against the sparse
+    Galerkin matrix K (on the free DOFs fr of an nd-vector)."""
+    fm = np.zeros(nd, bool)
+    fm[fr] = True
+    u = np.zeros(nd)
+    u[fr] = rng.standard_normal(fr.size)
+    Au = np.zeros(nd)
+    Au[fr] = K @ u[fr]
+    rhs = np.where(fm, rng.standard_normal(nd), 0.0)
+    dinv = rng.uniform(0.5, 2.0, nd)
+    d0 = np.where(fm, rng.standard_normal(nd), 0.0)
+    x0 = np.where(fm, rng.standard_normal(nd), 0.0)
+    f32 = lambda a: t(a.astype(np.float32))   # noqa: E731
+    du = f32(u)
+    tol = 2e-6
+    y = torch.zeros(nd, dtype=torch.float32, device=DEV)
+    dot = torch.zeros(1, dtype=torch.float64, device=DEV)
+    L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_STORE), stream())
+    assert rel_l2(h(y), Au) < tol
+    L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), stream())
+    assert h(dot)[0] == pytest.approx(float(u @ Au), rel=tol)
+    L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_RESID_NORM, rhs=f32(rhs), dot=dot), stream())
+    r = rhs - Au
+    assert rel_l2(h(y), r) < tol and h(dot)[0] == pytest.approx(float(r @ r), rel=tol)
+    a1, a2 = 0.7, 0.2
+    dd, xo = f32(d0), torch.zeros(nd, dtype=torch.float32, device=DEV)
+    L.call("tg_apply32", op, L.ptr(du), None,
+           L.ep(L.TG_EP_CHEB, rhs=f32(rhs), dinv=f32(dinv), d=dd, x=xo, a1=a1, a2=a2), stream())
+    dn = np.where(fm, a1 * dinv * r + a2 * d0, 0.0)
+    assert rel_l2(h(dd), dn) < tol and rel_l2(h(xo), np.where(fm, u + dn, 0.0)) < tol
+    dr, dx, dout = f32(rhs), f32(x0), torch.zeros(nd, dtype=torch.float32, device=DEV)
+    L.call("tg_apply32", op, L.ptr(du), None,
+           L.ep(L.TG_EP_CHEB_R, dinv=f32(dinv), d=dout, x=dx, r=dr, a1=a1, a2=a2, a3=1.0), stream())
+    dn = np.where(fm, a1 * dinv * r + a2 * u, 0.0)
+    assert rel_l2(h(dr), r) < tol and rel_l2(h(dout), dn) < tol
+    assert rel_l2(h(dx), np.where(fm, x0 + u + dn, 0.0)) < tol
+    dr = f32(rhs)
+    L.call("tg_apply32", op, L.ptr(du), None, L.ep(L.TG_EP_RESID_UPDATE, r=dr), stream())
+    assert rel_l2(h(dr), r) < tol
+
+
 @pytest.mark.parametrize("dims,point", [((8, 4, 4), False), ((8, 4, 4), True), ((7, 5, 6), False),
                                         ((12, 6, 3), False)])
 def test_galerkin_levels_match_ptkp(dims, point):
@@ -251,6 +302,9 @@ def test_galerkin_levels_match_ptkp(dims, point):
         u[fr] = rng.standard_normal(fr.size)
         y = _apply_stored(dc, masks[lv], m32, u)
         assert rel_l2(y[fr], Kc @ u[fr]) < 5e-6, lv
+        op_s, coef, dm_s = _stencil_op(dc, masks[lv], m64)
+        assert not torch.isnan(coef).any()
+        _check_stencil_epilogues(op_s, Kc, fr, nd, rng)
         assert np.array_equal(fem.free_from_node_mask(masks[lv]), fr)
         if lv == 1 and not point and all(d % 2 == 0 for d in dims):
             yq = torch.zeros(nd, dtype=torch.float32, device=DEV)
@@ -315,8 +369,34 @@ def test_galerkin_from_fine_matches_two_step(dims):
 
 def test_not_spd_is_reported():
     A = t(np.array([1.0, 2.0, 2.0, 1.0]))
-    with pytest.raises(L.TopoHipError, match="Cholesky"):
+    with pytest.raises(L.TopoHipError, match="not SPD"):
         L.call("tg_dense_spd_inverse", L.ptr(A), 2, stream())
+    n = 400   # library path
+    B = np.eye(n)
+    B[n - 1, n - 1] = -1.0
+    with pytest.raises(L.TopoHipError, match="not SPD"):
+        L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), n, stream())
+
+
+@pytest.mark.parametrize("n", [1, 7, 135, 320, 321, 600])
+def test_dense_spd_inverse_sizes(n):
+    """One-workgroup Gauss-Jordan (n <= 320) and library Cholesky (n > 320)
+    invert an ill-conditioned SPD matrix to FP64 accuracy, symmetric."""
+    rng = np.random.default_rng(n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.logspace(0, 8, n)   # condition number 1e8
+    A = (Q * ev) @ Q.T
+    A = 0.5 * (A + A.T)
+    d = t(A.ravel())
+    L.call("tg_dense_spd_inverse", L.ptr(d), n, stream())
+    Ai = h(d).reshape(n, n)
+    assert np.array_equal(Ai, Ai.T)
+    ref = np.linalg.inv(A)
+    assert np.abs(Ai - ref).max() < 1e-6 * np.abs(ref).max()
+    # forward error of a coarse solve (cond 1e8: both inverses err ~1e-8)
+    b = rng.standard_normal((n, 4))
+    x = np.linalg.solve(A, b)
+    assert np.linalg.norm(Ai @ b - x) < 1e-6 * np.linalg.norm(x)
 
 
 @pytest.mark.parametrize("rmin", [1.5, 2.0, 2.7])

@@ -252,8 +252,8 @@ extern "C" int tg_init(const double* ke) {
     void* p = nullptr;
     TG_HIP_CHECK(hipMalloc(&p, sizeof(double) * kMaxPartials + 256));
     g_ws[dev].partials = (double*)p;
-    g_ws[dev].ticket = (unsigned*)((char*)p + sizeof(double) * kMaxPartials);
-    TG_HIP_CHECK(hipMemset(g_ws[dev].ticket, 0, 256));
+    g_ws[dev].flags = (int*)((char*)p + sizeof(double) * kMaxPartials);
+    TG_HIP_CHECK(hipMemset(g_ws[dev].flags, 0, 256));
   }
   TG_HIP_CHECK(hipDeviceSynchronize());
   g_ready[dev] = true;

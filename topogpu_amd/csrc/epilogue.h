@@ -40,25 +40,64 @@ struct EpMode {
   };
   static constexpr bool kReduces = MODE == TG_EP_DOT || MODE == TG_EP_RESID_NORM;
 
+  // One component c of a node: fetch the mode's inputs.
+  template <class Acc>
+  __device__ __forceinline__ static void load_one(const Ep32& e, const Acc& a, int c, float& p, float& q, float& s) {
+    p = q = s = 0.0f;
+    if constexpr (MODE == TG_EP_RESID || MODE == TG_EP_RESID_NORM) {
+      p = a.template ld<EV_RHS>(c);
+    } else if constexpr (MODE == TG_EP_CHEB) {
+      p = a.template ld<EV_RHS>(c);
+      q = a.template ld<EV_DINV>(c);
+      if (e.a2 != 0.0f) s = a.template ld<EV_D>(c);
+    } else if constexpr (MODE == TG_EP_CHEB_R) {
+      p = a.template ld<EV_R>(c);
+      q = a.template ld<EV_DINV>(c);
+      s = a.template ld<EV_X>(c);
+    } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
+      p = a.template ld<EV_R>(c);
+    }
+  }
+
+  // One component: combine A u (au) with the inputs, apply the free bit f,
+  // write; returns the component's share of the reduction.
+  template <class Acc>
+  __device__ __forceinline__ static double store_one(const Ep32& e, const Acc& a, int c, bool f, float au, float u,
+                                                     float p, float q, float s) {
+    double part = 0.0;
+    if constexpr (MODE == TG_EP_STORE) {
+      a.template st<EV_Y>(c, f ? au : 0.0f);
+    } else if constexpr (MODE == TG_EP_DOT) {
+      const float v = f ? au : 0.0f;
+      a.template st<EV_Y>(c, v);
+      part = (double)u * (double)v;
+    } else if constexpr (MODE == TG_EP_RESID) {
+      a.template st<EV_Y>(c, f ? p - au : 0.0f);
+    } else if constexpr (MODE == TG_EP_RESID_NORM) {
+      const float v = f ? p - au : 0.0f;
+      a.template st<EV_Y>(c, v);
+      part = (double)v * (double)v;
+    } else if constexpr (MODE == TG_EP_CHEB) {
+      const float t = q * (p - au);
+      const float dn = f ? ((e.a2 != 0.0f) ? e.a1 * t + e.a2 * s : e.a1 * t) : 0.0f;
+      a.template st<EV_D>(c, dn);
+      a.template st<EV_X>(c, f ? u + dn : 0.0f);
+    } else if constexpr (MODE == TG_EP_CHEB_R) {
+      const float rn = p - au;
+      const float dn = e.a1 * q * rn + e.a2 * u;
+      a.template st<EV_R>(c, f ? rn : 0.0f);
+      a.template st<EV_D>(c, f ? dn : 0.0f);
+      a.template st<EV_X>(c, f ? s + e.a3 * u + dn : 0.0f);
+    } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
+      a.template st<EV_R>(c, f ? p - au : 0.0f);
+    }
+    return part;
+  }
+
   template <class Acc>
   __device__ __forceinline__ static void load(const Ep32& e, const Acc& a, In& in) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      in.p[c] = in.q[c] = in.s[c] = 0.0f;
-      if constexpr (MODE == TG_EP_RESID || MODE == TG_EP_RESID_NORM) {
-        in.p[c] = a.template ld<EV_RHS>(c);
-      } else if constexpr (MODE == TG_EP_CHEB) {
-        in.p[c] = a.template ld<EV_RHS>(c);
-        in.q[c] = a.template ld<EV_DINV>(c);
-        if (e.a2 != 0.0f) in.s[c] = a.template ld<EV_D>(c);
-      } else if constexpr (MODE == TG_EP_CHEB_R) {
-        in.p[c] = a.template ld<EV_R>(c);
-        in.q[c] = a.template ld<EV_DINV>(c);
-        in.s[c] = a.template ld<EV_X>(c);
-      } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
-        in.p[c] = a.template ld<EV_R>(c);
-      }
-    }
+    for (int c = 0; c < 3; ++c) load_one(e, a, c, in.p[c], in.q[c], in.s[c]);
   }
 
   template <class Acc>
@@ -66,35 +105,7 @@ struct EpMode {
                                                  const float (&u)[3], const In& in) {
     double part = 0.0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const bool f = (m >> c) & 1u;
-      if constexpr (MODE == TG_EP_STORE) {
-        a.template st<EV_Y>(c, f ? au[c] : 0.0f);
-      } else if constexpr (MODE == TG_EP_DOT) {
-        const float v = f ? au[c] : 0.0f;
-        a.template st<EV_Y>(c, v);
-        part += (double)u[c] * (double)v;
-      } else if constexpr (MODE == TG_EP_RESID) {
-        a.template st<EV_Y>(c, f ? in.p[c] - au[c] : 0.0f);
-      } else if constexpr (MODE == TG_EP_RESID_NORM) {
-        const float v = f ? in.p[c] - au[c] : 0.0f;
-        a.template st<EV_Y>(c, v);
-        part += (double)v * (double)v;
-      } else if constexpr (MODE == TG_EP_CHEB) {
-        const float t = in.q[c] * (in.p[c] - au[c]);
-        const float dn = f ? ((e.a2 != 0.0f) ? e.a1 * t + e.a2 * in.s[c] : e.a1 * t) : 0.0f;
-        a.template st<EV_D>(c, dn);
-        a.template st<EV_X>(c, f ? u[c] + dn : 0.0f);
-      } else if constexpr (MODE == TG_EP_CHEB_R) {
-        const float rn = in.p[c] - au[c];
-        const float dn = e.a1 * in.q[c] * rn + e.a2 * u[c];
-        a.template st<EV_R>(c, f ? rn : 0.0f);
-        a.template st<EV_D>(c, f ? dn : 0.0f);
-        a.template st<EV_X>(c, f ? in.s[c] + e.a3 * u[c] + dn : 0.0f);
-      } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
-        a.template st<EV_R>(c, f ? in.p[c] - au[c] : 0.0f);
-      }
-    }
+    for (int c = 0; c < 3; ++c) part += store_one(e, a, c, (m >> c) & 1u, au[c], u[c], in.p[c], in.q[c], in.s[c]);
     return part;
   }
 };

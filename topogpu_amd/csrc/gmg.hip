@@ -214,6 +214,44 @@ __global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const i
   if (lane == 0) ec[fl[row]] = (float)s;
 }
 
+// In-place inverse of a symmetric positive definite n x n matrix (row-major)
+// by Gauss-Jordan elimination without pivoting, in one workgroup. The pivots
+// of Gauss-Jordan on an SPD matrix are the Cholesky pivots squared, so they
+// stay positive; a pivot <= 0 (or NaN) reports "not SPD" through info.
+// Step k: cache row k (scaled by 1/pivot) and column k in LDS, then update
+// every entry from the two cached vectors. Cost n^3 FMA on one CU, which is
+// less than the launch sequence of a library factorisation for n <= 320.
+constexpr int GJ_THREADS = 1024, GJ_MAXN = 320;
+
+__global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(double* A, int n, int* info) {
+  __shared__ double prow[GJ_MAXN];
+  __shared__ double pcol[GJ_MAXN];
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+  if (tid == 0) *info = 0;
+  for (int k = 0; k < n; ++k) {
+    const double p = A[(long long)k * n + k];   // uniform across the block
+    if (!(p > 0.0)) {
+      if (tid == 0) *info = k + 1;
+      return;
+    }
+    const double ip = 1.0 / p;
+    for (int j = tid; j < n; j += GJ_THREADS) {
+      pcol[j] = A[(long long)j * n + k];
+      prow[j] = (j == k ? 1.0 : A[(long long)k * n + j]) * ip;
+    }
+    __syncthreads();
+    for (int i = ty; i < n; i += 32) {
+      const double f = pcol[i];
+      double* row = A + (long long)i * n;
+      for (int j = tx; j < n; j += 32) {
+        if (i == k) row[j] = prow[j];
+        else row[j] = (j == k ? 0.0 : row[j]) - f * prow[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 struct SolverHandles {
   rocblas_handle h = nullptr;
 };
@@ -389,6 +427,21 @@ extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const in
 
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
+  if (n <= GJ_MAXN) {
+    TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device");
+    int* info_d = workspace().flags;
+    hipLaunchKernelGGL(k_spd_inverse_gj, dim3(1), dim3(GJ_THREADS), 0, (hipStream_t)s, A, n, info_d);
+    TG_LAUNCH_CHECK();
+    int info = 0;
+    TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
+    TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+    if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
+    const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
+    hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)s,
+                       A, n);
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+  }
   rocblas_handle h = nullptr;
   if (int r = get_handle((hipStream_t)s, &h)) return r;
   rocblas_int* dinfo = nullptr;
@@ -410,7 +463,7 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   }
   TG_HIP_CHECK(hipFreeAsync(dinfo, (hipStream_t)s));
   if (st != rocblas_status_success) return set_error(TG_ERR_HIP, "rocsolver potrf/potri status %d", (int)st);
-  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense Cholesky failed (info=%d)", (int)info);
+  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (Cholesky info=%d)", (int)info);
   const long long nn = (long long)n * n;
   hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)s, A,
                      n);

@@ -21,6 +21,8 @@ namespace tg {
 int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const float* u, const Ep32& ep,
                     double* dot, bool split, hipStream_t s);
 int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s);
+int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
+                   double* dot, hipStream_t s);
 
 namespace {
 
@@ -329,6 +331,9 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
         return E::reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
       });
     }
+    case TG_OP_STENCIL_F32:
+      TG_REQUIRE(op->stencil, "stencil NULL");
+      return launch_stencil(g, op->mask, op->stencil, u, e, ep->dot, s);
     default:
       return set_error(TG_ERR_ARG, "unknown operator kind %d", op->kind);
   }

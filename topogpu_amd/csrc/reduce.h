@@ -15,7 +15,7 @@ namespace tg {
 
 struct Workspace {
   double* partials;     // >= kMaxPartials doubles
-  unsigned* ticket;     // spare counter word (kept zero)
+  int* flags;           // 64 device status words (flags[0]: dense inverse info)
 };
 constexpr int kMaxPartials = 1 << 19;   // 4 MiB of FP64 partials
 Workspace workspace();                  // per current device, valid after tg_init

@@ -8,8 +8,9 @@ Restates the hierarchy of ``GalerkinMatFreeGMG`` (src/gpu_fem/multigrid_v4.py:
   and the Dirichlet masks are aligned with the coarse grid its element
   matrices are sum_s E_child(s) * Q_s and are applied implicitly from the
   fine E (no stored matrices on the V-cycle path), otherwise stored;
-* levels >= 2 are stored Galerkin element matrices built on the device from
-  the level above (replacing the SpGEMM R @ (K @ P), multigrid_v4.py:747-752);
+* levels >= 2 are Galerkin element matrices built on the device from the
+  level above (replacing the SpGEMM R @ (K @ P), multigrid_v4.py:747-752),
+  assembled into 27-point block stencils for the V-cycle;
 * the coarsest level is solved with an FP64 dense Cholesky factorisation
   turned into an explicit inverse once per setup (multigrid_v4.py:774-811).
 
@@ -41,7 +42,7 @@ class _Level:
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
-                L.TG_OP_STORED_F32: "stored_f32"}
+                L.TG_OP_STORED_F32: "stored_f32", L.TG_OP_STENCIL_F32: "stencil_f32"}
 
 _EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
                    L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8}
@@ -103,14 +104,14 @@ class GMG:
             Lv.v0 = torch.from_numpy(rng.standard_normal(Lv.n).astype(np.float32)).to(device)
             Lv.lam = float("nan")
             Lv.mats64 = None
-            Lv.mats32 = None
+            Lv.stencil = None
             self.levels.append(Lv)
         self.levels[0].kind = fine_op
         self.levels[0].kind_accurate = fine_accurate_op if fine_accurate_op is not None else fine_op
         for lv in range(1, n_levels):
             f, c = self.levels[lv - 1], self.levels[lv]
             c.consistent = fem.masks_consistent(f.dims, c.dims, f.mask_host, c.mask_host)
-            c.kind = L.TG_OP_STORED_F32
+            c.kind = L.TG_OP_STENCIL_F32
         l1 = self.levels[1]
         if n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
             l1.kind = L.TG_OP_Q8_F32
@@ -149,9 +150,9 @@ class GMG:
     def op(self, lv: int, accurate: bool = False) -> L.TgOp32:
         Lv = self.levels[lv]
         kind = Lv.kind_accurate if accurate else Lv.kind
-        if kind == L.TG_OP_STORED_F32:
-            return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), None, L.ptr(Lv.mats32))
-        return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), L.ptr(self.E32), None)
+        if kind == L.TG_OP_STENCIL_F32:
+            return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), None, None, L.ptr(Lv.stencil))
+        return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), L.ptr(self.E32), None, None)
 
     def _launch(self, lv, op, u, y, ep, stream):
         Lv = self.levels[lv]
@@ -173,10 +174,7 @@ class GMG:
             else:
                 if c.mats64 is None:
                     c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
-                need32 = c.kind == L.TG_OP_STORED_F32 and lv < last_lv
-                if need32 and c.mats32 is None:
-                    c.mats32 = torch.empty(576 * c.n_elem, dtype=torch.float32, device=self.dev)
-                m32 = L.ptr(c.mats32) if need32 else None
+                m32 = None
                 lv0 = self.levels[0]
                 if lv == 1 and self.l1_from_fine:
                     L.call("tg_galerkin_from_fine", lv0.grid, 1, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
@@ -192,6 +190,9 @@ class GMG:
                 if c.kind == L.TG_OP_Q8_F32:
                     L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
                 else:
+                    if c.stencil is None:
+                        c.stencil = torch.empty(243 * c.nodes, dtype=torch.float32, device=self.dev)
+                    L.call("tg_assemble_stencil", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.stencil), stream)
                     L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
         last = self.levels[-1]
         L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,

@@ -13,7 +13,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libtopohip.so"
 
-TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32 = 1, 2, 3, 4, 5
+TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32, TG_OP_STENCIL_F32 = 1, 2, 3, 4, 5, 6
 TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE = range(7)
 OC_STATE_LEN = 32   # device doubles of the OC bisection state (TG_OC_STATE_LEN)
 
@@ -29,7 +29,8 @@ class TgGrid(C.Structure):
 
 
 class TgOp32(C.Structure):
-    _fields_ = [("kind", _i32), ("grid", TgGrid), ("mask", _vp), ("weights", _vp), ("elem_mats", _vp)]
+    _fields_ = [("kind", _i32), ("grid", TgGrid), ("mask", _vp), ("weights", _vp), ("elem_mats", _vp),
+                ("stencil", _vp)]
 
 
 class TgEp32(C.Structure):
@@ -57,6 +58,7 @@ _SIGS = {
     "tg_galerkin_elems": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, C.c_int, _vp, _vp, _vp, _vp, _vp],
     "tg_galerkin_from_fine": [C.POINTER(TgGrid), C.c_int, _vp, _vp, _vp, _vp, _vp],
     "tg_assemble_dense": [C.POINTER(TgGrid), _vp, _vp, _i32, _vp, _vp],
+    "tg_assemble_stencil": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_dense_spd_inverse": [_vp, _i32, _vp],
     "tg_coarsest_apply": [_vp, _i32, _vp, _i64, _vp, _vp, _vp],
     "tg_dot32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp],
