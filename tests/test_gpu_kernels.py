@@ -378,9 +378,9 @@ def test_not_spd_is_reported():
         L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), n, stream())
 
 
-@pytest.mark.parametrize("n", [1, 7, 135, 320, 321, 600])
+@pytest.mark.parametrize("n", [1, 7, 135, 192, 193, 600])
 def test_dense_spd_inverse_sizes(n):
-    """One-workgroup Gauss-Jordan (n <= 320) and library Cholesky (n > 320)
+    """One-workgroup Gauss-Jordan (n <= 192) and library Cholesky (n > 192)
     invert an ill-conditioned SPD matrix to FP64 accuracy, symmetric."""
     rng = np.random.default_rng(n)
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))

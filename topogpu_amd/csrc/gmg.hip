@@ -216,40 +216,81 @@ __global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const i
 
 // In-place inverse of a symmetric positive definite n x n matrix (row-major)
 // by Gauss-Jordan elimination without pivoting, in one workgroup. The pivots
-// of Gauss-Jordan on an SPD matrix are the Cholesky pivots squared, so they
-// stay positive; a pivot <= 0 (or NaN) reports "not SPD" through info.
-// Step k: cache row k (scaled by 1/pivot) and column k in LDS, then update
-// every entry from the two cached vectors. Cost n^3 FMA on one CU, which is
-// less than the launch sequence of a library factorisation for n <= 320.
-constexpr int GJ_THREADS = 1024, GJ_MAXN = 320;
+// of Gauss-Jordan on an SPD matrix are the pivots of its Cholesky
+// factorisation squared, so they stay positive; a pivot <= 0 (or NaN)
+// reports "not SPD" through info.
+// The matrix lives in registers: thread (tr, tc) = (tid / 32, tid % 32) owns
+// the entries (tr + 32a, tc + 32b), a, b < GJ_NB. Step k publishes row k and
+// column k through LDS and every thread updates its own entries, so each of
+// the n steps costs two barriers and a few FMAs per thread, with no
+// global-memory round trip.
+constexpr int GJ_THREADS = 1024, GJ_NB = 6;
+constexpr int GJ_MAXN = 32 * GJ_NB;   // 192
 
 __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(double* A, int n, int* info) {
   __shared__ double prow[GJ_MAXN];
   __shared__ double pcol[GJ_MAXN];
-  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
-  if (tid == 0) *info = 0;
+  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
+  double a[GJ_NB][GJ_NB];
+#pragma unroll
+  for (int x = 0; x < GJ_NB; ++x)
+#pragma unroll
+    for (int y = 0; y < GJ_NB; ++y) {
+      const int i = tr + 32 * x, j = tc + 32 * y;
+      a[x][y] = (i < n && j < n) ? A[(long long)i * n + j] : 0.0;
+    }
   for (int k = 0; k < n; ++k) {
-    const double p = A[(long long)k * n + k];   // uniform across the block
+    const int kq = k >> 5, kr = k & 31;
+    // column kq / row kq of the register tile, selected without dynamic
+    // register indexing (which would move the tile to scratch memory)
+    if (tc == kr)
+#pragma unroll
+      for (int x = 0; x < GJ_NB; ++x) {
+        double v = 0.0;
+#pragma unroll
+        for (int y = 0; y < GJ_NB; ++y) v = (y == kq) ? a[x][y] : v;
+        pcol[tr + 32 * x] = v;
+      }
+    if (tr == kr)
+#pragma unroll
+      for (int y = 0; y < GJ_NB; ++y) {
+        double v = 0.0;
+#pragma unroll
+        for (int x = 0; x < GJ_NB; ++x) v = (x == kq) ? a[x][y] : v;
+        prow[tc + 32 * y] = v;
+      }
+    __syncthreads();
+    const double p = pcol[k];   // uniform
     if (!(p > 0.0)) {
       if (tid == 0) *info = k + 1;
       return;
     }
     const double ip = 1.0 / p;
-    for (int j = tid; j < n; j += GJ_THREADS) {
-      pcol[j] = A[(long long)j * n + k];
-      prow[j] = (j == k ? 1.0 : A[(long long)k * n + j]) * ip;
+    double pr[GJ_NB], pc[GJ_NB];
+#pragma unroll
+    for (int y = 0; y < GJ_NB; ++y) {
+      const int j = tc + 32 * y;
+      pr[y] = (j == k ? 1.0 : prow[j]) * ip;
     }
-    __syncthreads();
-    for (int i = ty; i < n; i += 32) {
-      const double f = pcol[i];
-      double* row = A + (long long)i * n;
-      for (int j = tx; j < n; j += 32) {
-        if (i == k) row[j] = prow[j];
-        else row[j] = (j == k ? 0.0 : row[j]) - f * prow[j];
+#pragma unroll
+    for (int x = 0; x < GJ_NB; ++x) pc[x] = pcol[tr + 32 * x];
+#pragma unroll
+    for (int x = 0; x < GJ_NB; ++x)
+#pragma unroll
+      for (int y = 0; y < GJ_NB; ++y) {
+        const int i = tr + 32 * x, j = tc + 32 * y;
+        a[x][y] = (i == k) ? pr[y] : (j == k ? 0.0 : a[x][y]) - pc[x] * pr[y];
       }
-    }
     __syncthreads();
   }
+#pragma unroll
+  for (int x = 0; x < GJ_NB; ++x)
+#pragma unroll
+    for (int y = 0; y < GJ_NB; ++y) {
+      const int i = tr + 32 * x, j = tc + 32 * y;
+      if (i < n && j < n) A[(long long)i * n + j] = a[x][y];
+    }
+  if (tid == 0) *info = 0;
 }
 
 struct SolverHandles {

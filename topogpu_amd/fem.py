@@ -66,13 +66,13 @@ def level_dims(dims, n_levels: int):
     return out
 
 
-def auto_levels(dims, mask0: np.ndarray, max_coarse: int = 300, max_levels: int = 12) -> int:
+def auto_levels(dims, mask0: np.ndarray, max_coarse: int = 192, max_levels: int = 12) -> int:
     """Fewest levels (>= 2) whose coarsest level has at most ``max_coarse``
     free DOFs. The coarsest level is inverted densely once per design
-    iteration; up to ~300 unknowns this is one workgroup-sized kernel
-    (Gauss-Jordan, n^3 FMA), while a larger coarsest level needs a library
-    factorisation whose launch sequence costs milliseconds, more than the
-    smoothing of the extra levels."""
+    iteration; up to 192 unknowns this is one workgroup holding the matrix in
+    registers (Gauss-Jordan, n^3 FMA), while a larger coarsest level needs a
+    library factorisation whose launch sequence costs milliseconds, more than
+    the smoothing of the extra levels."""
     dims_l, mask = tuple(dims), mask0
     for n_levels in range(2, max_levels + 1):
         nxt = coarse_dims(dims_l)

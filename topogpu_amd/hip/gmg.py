@@ -68,7 +68,7 @@ class GMG:
     def __init__(self, dims, mask0: np.ndarray, *, n_levels: int = 4, fine_op: int = L.TG_OP_HEX_BF16,
                  fine_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
-                 power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7):
+                 power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8"):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
         self.dev = device
@@ -113,7 +113,9 @@ class GMG:
             c.consistent = fem.masks_consistent(f.dims, c.dims, f.mask_host, c.mask_host)
             c.kind = L.TG_OP_STENCIL_F32
         l1 = self.levels[1]
-        if n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
+        if level1 not in ("q8", "stencil"):
+            raise ValueError(f"level1 must be 'q8' or 'stencil'; got {level1!r}")
+        if level1 == "q8" and n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
             l1.kind = L.TG_OP_Q8_F32
         for lv in range(1, n_levels):
             self.levels[lv].kind_accurate = self.levels[lv].kind

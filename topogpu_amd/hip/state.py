@@ -61,7 +61,7 @@ class HipStateSolver:
                  fine_degree: int = 2, coarse_degree: int = 2, cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
-                 warm_start: bool = True, device=None):
+                 warm_start: bool = True, level1: str = "q8", device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -90,7 +90,7 @@ class HipStateSolver:
         self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=PRECISIONS[precision][1],
                        fine_accurate_op=PRECISIONS[precision][0], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
-                       device=self.dev)
+                       level1=level1, device=self.dev)
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.u = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
         self.r64 = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
