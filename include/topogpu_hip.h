@@ -70,7 +70,11 @@ enum {
   /* assembled 27-point block stencil from tg_assemble_stencil (243 FP32 per
    * node); input entries at fixed DOFs must be finite (their coefficients
    * are zero)                                                              */
-  TG_OP_STENCIL_F32 = 6
+  TG_OP_STENCIL_F32 = 6,
+  /* as TG_OP_Q8_F32 with the eight child products on BF16 MFMA (FP32
+   * accumulate), single pass / hi-lo split operands                        */
+  TG_OP_Q8_BF16 = 7,
+  TG_OP_Q8_BF16X3 = 8
 };
 
 typedef struct {

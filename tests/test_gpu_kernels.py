@@ -236,9 +236,9 @@ def _stencil_op(dims, mask, m64):
     return L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dims), L.ptr(dm), None, None, L.ptr(coef)), coef, dm
 
 
-def _check_stencil_epilogues(op, K, fr, nd, rng):
-    """Every epilogue of the assembled-stencil operator against the sparse
-    Galerkin matrix K (on the free DOFs fr of an nd-vector)."""
+def _check_stencil_epilogues(op, K, fr, nd, rng, tol=2e-6):
+    """Every epilogue of a coarse-level operator against the sparse Galerkin
+    matrix K (on the free DOFs fr of an nd-vector)."""
     fm = np.zeros(nd, bool)
     fm[fr] = True
     u = np.zeros(nd)
@@ -251,7 +251,6 @@ def _check_stencil_epilogues(op, K, fr, nd, rng):
     x0 = np.where(fm, rng.standard_normal(nd), 0.0)
     f32 = lambda a: t(a.astype(np.float32))   # noqa: E731
     du = f32(u)
-    tol = 2e-6
     y = torch.zeros(nd, dtype=torch.float32, device=DEV)
     dot = torch.zeros(1, dtype=torch.float64, device=DEV)
     L.call("tg_apply32", op, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_STORE), stream())
@@ -279,7 +278,7 @@ def _check_stencil_epilogues(op, K, fr, nd, rng):
 
 
 @pytest.mark.parametrize("dims,point", [((8, 4, 4), False), ((8, 4, 4), True), ((7, 5, 6), False),
-                                        ((12, 6, 3), False)])
+                                        ((12, 6, 3), False), ((26, 18, 34), False)])
 def test_galerkin_levels_match_ptkp(dims, point):
     free, F, ndof, E, mask0, rng = _setup(dims, 6, point_support=point)
     levels = _oracle_levels(dims, free, E, 3)
@@ -312,6 +311,11 @@ def test_galerkin_levels_match_ptkp(dims, point):
             L.call("tg_apply32", op, L.ptr(t(u.astype(np.float32))), L.ptr(yq),
                    L.ep(L.TG_EP_STORE), stream())
             assert rel_l2(h(yq)[fr], Kc @ u[fr]) < 5e-6
+            # the same operator with the child products on BF16 MFMA
+            dE32, dm1 = t(E.astype(np.float32)), t(masks[1])
+            for kind, tol in ((L.TG_OP_Q8_BF16X3, 2e-5), (L.TG_OP_Q8_BF16, 2e-2)):
+                opm = L.TgOp32(kind, L.grid(dc), L.ptr(dm1), L.ptr(dE32), None)
+                _check_stencil_epilogues(opm, Kc, fr, nd, rng, tol=tol)
             d32 = torch.zeros(nd, dtype=torch.float32, device=DEV)
             L.call("tg_diag_inv_q8", L.grid(dc), L.ptr(t(masks[1])), L.ptr(t(E)), L.ptr(d32), stream())
             assert rel_inf(h(d32)[fr], 1.0 / Kc.diagonal()) < 1e-6

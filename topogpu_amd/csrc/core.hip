@@ -16,6 +16,9 @@ __constant__ float c_q8_32[8 * 576];
 // BF16 bit patterns of the unit stiffness in MFMA A-operand order, see
 // mfma_layout.h: [output layout][hi/lo][M tile][lane][8].
 __constant__ uint16_t c_ke_bf16[2 * 2 * 2 * 64 * 8];
+// The eight Galerkin child matrices Q_s in the same A-operand order:
+// [child s][output layout][hi/lo][M tile][lane][8] (global memory: 128 KiB).
+__device__ uint16_t c_q8_bf16[8 * 2 * 2 * 2 * 64 * 8];
 
 namespace {
 thread_local char g_err[512] = "";
@@ -242,6 +245,23 @@ extern "C" int tg_init(const double* ke) {
           kb[(((ly * 2 + 0) * 2 + t) * 64 + l) * 8 + v] = hi;
           kb[(((ly * 2 + 1) * 2 + t) * 64 + l) * 8 + v] = lo;
         }
+
+  static uint16_t qb[8 * 2 * 2 * 2 * 64 * 8];
+  for (int sc = 0; sc < 8; ++sc)
+    for (int ly = 0; ly < 2; ++ly)
+      for (int t = 0; t < 2; ++t)
+        for (int l = 0; l < 64; ++l)
+          for (int v = 0; v < 8; ++v) {
+            const int row = mfma_a_row(t, l) & 15, col = mfma_a_col(l, v);
+            const int g = row >> 2, cr = row & 3, nc = col >> 2, cc = col & 3;
+            const int nr = corner_of(mfma_out_cx(g), mfma_out_cy(ly, g), t);
+            const float x = (cr < 3 && cc < 3) ? q8[sc * 576 + (3 * nr + cr) * 24 + 3 * nc + cc] : 0.0f;
+            const uint16_t hi = bf16_bits_rne(x);
+            const uint16_t lo = bf16_bits_rne(x - bf16_to_float(hi));
+            qb[((((sc * 2 + ly) * 2 + 0) * 2 + t) * 64 + l) * 8 + v] = hi;
+            qb[((((sc * 2 + ly) * 2 + 1) * 2 + t) * 64 + l) * 8 + v] = lo;
+          }
+  TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_q8_bf16), qb, sizeof(qb)));
 
   for (int i = 0; i < 576; ++i) g_ke_host[i] = ke[i];
   TG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ke64), ke, sizeof(double) * 576));

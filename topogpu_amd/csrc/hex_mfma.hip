@@ -1,18 +1,22 @@
-// Level-0 hex8 operator with the element GEMM on BF16 MFMA (gfx950).
+// Hex8 element operators with the element GEMM on BF16 MFMA (gfx950):
+//   k_hex_mfma : level 0, element matrix E_e * KE;
+//   k_q8_mfma  : Galerkin level 1 of a fine grid with even dims, element
+//                matrix sum_s E_child(s) * Q_s over the eight fine children.
 //
 // A workgroup of NW wavefronts owns a tile of MTX = NW-1 (x) by MTY (y) node
 // lines and MTZ = 15 z-nodes [k0, k0+15). It computes every element touching
 // those nodes: NW x (MTY+1) element columns, each a z-pencil of 16 elements
 // [k0-1, k0+15). Wavefront w takes the column at x offset w and walks its
 // MTY+1 pencils along y. For one pencil the 16 element products
-//     F(32 x 16) = KE'(32 x 32) * U(32 x 16)
-// are two v_mfma_f32_16x16x32_bf16 (rows/columns in node-pad order, see
-// mfma_layout.h), FP32 accumulate, then scaled by E in FP32. Before the BF16
-// cast each element's displacement is taken relative to its first node:
-// KE annihilates rigid translations, so this is exact in exact arithmetic and
-// makes the rounding error proportional to the element's deformation rather
-// than to its absolute displacement. The optional split mode adds the
-// hi*lo and lo*hi products of a two-term BF16 expansion (3 MFMA per tile).
+//     F(32 x 16) = A(32 x 32) * U(32 x 16)
+// are two v_mfma_f32_16x16x32_bf16 per matrix A (rows/columns in node-pad
+// order, see mfma_layout.h), FP32 accumulate, scaled by the element weight in
+// FP32. Before the BF16 cast each element's displacement is taken relative to
+// its first node: KE and every Q_s annihilate rigid translations, so this is
+// exact in exact arithmetic and makes the rounding error proportional to the
+// element's deformation rather than to its absolute displacement. The split
+// mode adds the hi*lo and lo*hi products of a two-term BF16 expansion
+// (3 MFMA per tile), which brings the product to FP32-level accuracy.
 //
 // Reduction without atomics:
 //  * z: lane (g, j) ends with the forces of element j on its corner-g node of
@@ -42,22 +46,32 @@
 namespace tg {
 
 extern __constant__ uint16_t c_ke_bf16[2 * 2 * 2 * 64 * 8];
+extern __device__ uint16_t c_q8_bf16[8 * 2 * 2 * 2 * 64 * 8];
 
 namespace {
 
-constexpr int NW = 8;                                       // wavefronts = element columns in x
-constexpr int NTHR = 64 * NW;
-constexpr int MTX = NW - 1, MTY = 8, MTZ = 15;              // owned node tile
-constexpr int PY = MTY + 1;                                 // pencils per wavefront
-constexpr int UBX = MTX + 2, UBY = MTY + 2, UBZ = MTZ + 2;  // node brick
-constexpr int SZ = 16;                                      // z records per slot side (15 used)
-constexpr uint32_t kOOB = 0x80000000u;                      // offset past every descriptor
+constexpr uint32_t kOOB = 0x80000000u;   // offset past every descriptor
 constexpr uint32_t kMaxRange = 0x7fffffffu;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+template <int NW_>
+struct TileT {
+  static constexpr int NW = NW_;                  // wavefronts = element columns in x
+  static constexpr int NTHR = 64 * NW;
+  static constexpr int MTX = NW - 1, MTY = 8, MTZ = 15;   // owned node tile
+  static constexpr int PY = MTY + 1;              // pencils per wavefront
+  static constexpr int UBX = MTX + 2, UBY = MTY + 2, UBZ = MTZ + 2;   // node brick
+  static constexpr int SZ = 16;                   // z records per slot side (15 used)
+  static constexpr int NB = UBX * UBY * UBZ, NIT = (NB + NTHR - 1) / NTHR;
+  static constexpr int NOUT = MTX * MTY * MTZ, OIT = (NOUT + NTHR - 1) / NTHR;
+  static constexpr int SLOT = MTX * MTY * 2 * SZ * 3;
+};
+using Tile0 = TileT<8>;   // level 0
+using Tile1 = TileT<4>;   // level 1 (8x fewer elements: smaller tiles keep the CUs busy)
 
 __device__ __forceinline__ rsrc_t make_rsrc(const void* base, unsigned long long bytes) {
   const int n = (int)(bytes < kMaxRange ? bytes : kMaxRange);
@@ -70,8 +84,14 @@ __device__ __forceinline__ float row_shift_up(float x) {
                                                                0xf, true));
 }
 
-__device__ __forceinline__ bf16x8 load_a(int layout, int hilo, int tile, int lane) {
+__device__ __forceinline__ bf16x8 load_ke(int layout, int hilo, int tile, int lane) {
   const uint4 w = *reinterpret_cast<const uint4*>(&c_ke_bf16[(((layout * 2 + hilo) * 2 + tile) * 64 + lane) * 8]);
+  return __builtin_bit_cast(bf16x8, w);
+}
+
+__device__ __forceinline__ bf16x8 load_q8(int s, int layout, int hilo, int tile, int lane) {
+  const uint4 w =
+      *reinterpret_cast<const uint4*>(&c_q8_bf16[((((s * 2 + layout) * 2 + hilo) * 2 + tile) * 64 + lane) * 8]);
   return __builtin_bit_cast(bf16x8, w);
 }
 
@@ -103,93 +123,209 @@ struct TileShape {
   int ntz, nty;   // tiles along z and y
 };
 
-template <bool SPLIT, int MODE>
-__global__ void __launch_bounds__(NTHR) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
-                                                  const float* __restrict__ E, const float* __restrict__ u,
-                                                  Ep32 ep, Workspace ws) {
-  using Mode = EpMode<MODE>;
-  __shared__ float4 su[UBX * UBY * UBZ];
-  // [x line][y line][side][z][3]: side 0 is the share of the element column
-  // on the -x side of the line, side 1 the +x side.
-  __shared__ float slot[MTX * MTY * 2 * SZ * 3];
+// Workgroup geometry and node-vector descriptors.
+struct Block {
+  int i0, j0, k0;    // owned tile origin (nodes)
+  int is;            // first x-plane of the slab
+  int NXn, NYn, NZn; // node dims
+  rsrc_t rs_m, rs_u;
+  BufAcc acc;
+};
 
-  const int NXn = g.nx + 1, NYn = g.ny + 1, NZn = g.nz + 1;
+template <class T>
+__device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const uint8_t* mask, const float* u,
+                                             const Ep32& ep) {
+  Block b;
+  b.NXn = g.nx + 1;
+  b.NYn = g.ny + 1;
+  b.NZn = g.nz + 1;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tz = t % ts.ntz, ty = (t / ts.ntz) % ts.nty, tx = t / (ts.ntz * ts.nty);
-  const int i0 = tx * MTX, j0 = ty * MTY, k0 = tz * MTZ;
-
-  // Slab of x-planes this workgroup touches, as descriptors with 32-bit
-  // offsets relative to the slab start.
-  const int is = i0 > 0 ? i0 - 1 : 0;
-  const long long nslab = (long long)is * NYn * NZn, eslab = (long long)is * g.ny * g.nz;
-  const long long nn = (long long)NXn * NYn * NZn, ne = (long long)g.nx * g.ny * g.nz;
-  const rsrc_t rs_m = make_rsrc(mask + nslab, (unsigned long long)(nn - nslab));
-  const rsrc_t rs_u = make_rsrc(u + 3 * nslab, 12ull * (nn - nslab));
-  const rsrc_t rs_e = make_rsrc(E + eslab, 4ull * (ne - eslab));
+  b.i0 = tx * T::MTX;
+  b.j0 = ty * T::MTY;
+  b.k0 = tz * T::MTZ;
+  b.is = b.i0 > 0 ? b.i0 - 1 : 0;
+  const long long nslab = (long long)b.is * b.NYn * b.NZn;
+  const long long nn = (long long)b.NXn * b.NYn * b.NZn;
+  b.rs_m = make_rsrc(mask + nslab, (unsigned long long)(nn - nslab));
+  b.rs_u = make_rsrc(u + 3 * nslab, 12ull * (nn - nslab));
   const unsigned long long vbytes = 12ull * (nn - nslab);
-  BufAcc acc;
-  {
-    const float* z = u;   // unused vectors: any valid base, never accessed
-    acc.rhs = make_rsrc((ep.rhs ? ep.rhs : z) + 3 * nslab, ep.rhs ? vbytes : 0);
-    acc.dinv = make_rsrc((ep.dinv ? ep.dinv : z) + 3 * nslab, ep.dinv ? vbytes : 0);
-    acc.d = make_rsrc((ep.d ? ep.d : z) + 3 * nslab, ep.d ? vbytes : 0);
-    acc.x = make_rsrc((ep.x_out ? ep.x_out : z) + 3 * nslab, ep.x_out ? vbytes : 0);
-    acc.r = make_rsrc((ep.r ? ep.r : z) + 3 * nslab, ep.r ? vbytes : 0);
-    acc.y = make_rsrc((ep.y ? ep.y : z) + 3 * nslab, ep.y ? vbytes : 0);
-  }
+  const float* z = u;   // unused vectors: any valid base, never accessed
+  b.acc.rhs = make_rsrc((ep.rhs ? ep.rhs : z) + 3 * nslab, ep.rhs ? vbytes : 0);
+  b.acc.dinv = make_rsrc((ep.dinv ? ep.dinv : z) + 3 * nslab, ep.dinv ? vbytes : 0);
+  b.acc.d = make_rsrc((ep.d ? ep.d : z) + 3 * nslab, ep.d ? vbytes : 0);
+  b.acc.x = make_rsrc((ep.x_out ? ep.x_out : z) + 3 * nslab, ep.x_out ? vbytes : 0);
+  b.acc.r = make_rsrc((ep.r ? ep.r : z) + 3 * nslab, ep.r ? vbytes : 0);
+  b.acc.y = make_rsrc((ep.y ? ep.y : z) + 3 * nslab, ep.y ? vbytes : 0);
+  return b;
+}
 
-  // ---- node brick: every load issued before any is used
-  constexpr int NB = UBX * UBY * UBZ, NIT = (NB + NTHR - 1) / NTHR;
-  uint32_t bw[NIT][3];
-  uint32_t bm[NIT];
+// Node brick loads: every load issued before any is used.
+template <class T>
+__device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT][3], uint32_t (&bm)[T::NIT]) {
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int idx = threadIdx.x + it * NTHR;
-    const int bz = idx % UBZ, line = idx / UBZ, by = line % UBY, bx = line / UBY;
-    const int i = i0 - 1 + bx, j = j0 - 1 + by, k = k0 - 1 + bz;
-    const bool in = idx < NB && (unsigned)i < (unsigned)NXn && (unsigned)j < (unsigned)NYn &&
-                    (unsigned)k < (unsigned)NZn;
-    const uint32_t rel = (uint32_t)(((i - is) * NYn + j) * NZn + k);
-    const auto w3 = __builtin_amdgcn_raw_buffer_load_b96(rs_u, in ? 12u * rel : kOOB, 0, 0);
-    bm[it] = __builtin_amdgcn_raw_buffer_load_b8(rs_m, in ? rel : kOOB, 0, 0);
+  for (int it = 0; it < T::NIT; ++it) {
+    const int idx = threadIdx.x + it * T::NTHR;
+    const int bz = idx % T::UBZ, line = idx / T::UBZ, by = line % T::UBY, bx = line / T::UBY;
+    const int i = b.i0 - 1 + bx, j = b.j0 - 1 + by, k = b.k0 - 1 + bz;
+    const bool in = idx < T::NB && (unsigned)i < (unsigned)b.NXn && (unsigned)j < (unsigned)b.NYn &&
+                    (unsigned)k < (unsigned)b.NZn;
+    const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
+    const auto w3 = __builtin_amdgcn_raw_buffer_load_b96(b.rs_u, in ? 12u * rel : kOOB, 0, 0);
+    bm[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, in ? rel : kOOB, 0, 0);
     bw[it][0] = w3[0];
     bw[it][1] = w3[1];
     bw[it][2] = w3[2];
   }
+}
 
-  // ---- owned output nodes: offsets and epilogue inputs, fetched now so
-  // their latency hides behind the pencil work
-  constexpr int NOUT = MTX * MTY * MTZ, OIT = (NOUT + NTHR - 1) / NTHR;
-  uint32_t ob[OIT];
-  uint32_t om[OIT];
-  typename Mode::In oin[OIT];
+// Masked node values into LDS (fixed DOFs read as zero).
+template <class T>
+__device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::NIT][3], const uint32_t (&bm)[T::NIT]) {
 #pragma unroll
-  for (int it = 0; it < OIT; ++it) {
-    const int idx = threadIdx.x + it * NTHR;
-    const int z = idx % MTZ, line = idx / MTZ, ti = line / MTY, tj = line % MTY;
-    const int i = i0 + ti, j = j0 + tj, k = k0 + z;
-    const bool own = idx < NOUT && i < NXn && j < NYn && k < NZn;
-    const uint32_t rel = (uint32_t)(((i - is) * NYn + j) * NZn + k);
+  for (int it = 0; it < T::NIT; ++it) {
+    const int idx = threadIdx.x + it * T::NTHR;
+    if (idx < T::NB)
+      su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
+                            (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
+                            (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
+  }
+}
+
+// Owned output nodes: offsets, masks and the epilogue inputs, fetched early
+// so their latency hides behind the pencil work.
+template <class T, int MODE>
+__device__ __forceinline__ void out_prefetch(Block& b, const Ep32& ep, uint32_t (&ob)[T::OIT], uint32_t (&om)[T::OIT],
+                                             typename EpMode<MODE>::In (&oin)[T::OIT]) {
+#pragma unroll
+  for (int it = 0; it < T::OIT; ++it) {
+    const int idx = threadIdx.x + it * T::NTHR;
+    const int z = idx % T::MTZ, line = idx / T::MTZ, ti = line / T::MTY, tj = line % T::MTY;
+    const int i = b.i0 + ti, j = b.j0 + tj, k = b.k0 + z;
+    const bool own = idx < T::NOUT && i < b.NXn && j < b.NYn && k < b.NZn;
+    const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
     ob[it] = own ? 12u * rel : kOOB;
-    om[it] = __builtin_amdgcn_raw_buffer_load_b8(rs_m, own ? rel : kOOB, 0, 0);
-    acc.b = ob[it];
-    Mode::load(ep, acc, oin[it]);
+    om[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, own ? rel : kOOB, 0, 0);
+    b.acc.b = ob[it];
+    EpMode<MODE>::load(ep, b.acc, oin[it]);
+  }
+}
+
+// Output stage: sum the two column shares of every owned node, apply the
+// epilogue; returns this thread's share of the grid reduction.
+template <class T, int MODE>
+__device__ __forceinline__ double out_store(Block& b, const Ep32& ep, const float* slot, const float4* su,
+                                            const uint32_t (&ob)[T::OIT], const uint32_t (&om)[T::OIT],
+                                            const typename EpMode<MODE>::In (&oin)[T::OIT]) {
+  double part = 0.0;
+#pragma unroll
+  for (int it = 0; it < T::OIT; ++it) {
+    int idx = threadIdx.x + it * T::NTHR;
+    idx = idx < T::NOUT ? idx : T::NOUT - 1;   // surplus threads read a valid slot; their stores are dropped
+    const int z = idx % T::MTZ, line = idx / T::MTZ, tli = line / T::MTY, tlj = line % T::MTY;
+    const float* s0 = &slot[(line * 2) * T::SZ * 3 + z * 3];
+    float au[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) au[v] = s0[v] + s0[T::SZ * 3 + v];
+    const float4 uo = su[((tli + 1) * T::UBY + tlj + 1) * T::UBZ + z + 1];
+    const float own[3] = {uo.x, uo.y, uo.z};
+    b.acc.b = ob[it];
+    part += EpMode<MODE>::store(ep, b.acc, om[it], au, own, oin[it]);
+  }
+  return part;
+}
+
+// Per-lane pencil addressing of wavefront w.
+template <class T>
+struct PencilLane {
+  int lane, je, grp;
+  int offA, offB, ubase;   // brick offsets of the lane's corners / element 0
+  bool wr_lo, wr_hi;       // slot write of a pair (py-1, py) with py-1 even / odd
+  int sbase;               // slot index of the lane's record (when it writes)
+
+  __device__ __forceinline__ PencilLane(int w) {
+    lane = threadIdx.x & 63;
+    je = lane & 15;     // element within the pencil (B and D column)
+    grp = lane >> 4;    // B: input corners 2grp, 2grp+1 ; D: output group
+    const int cA = 2 * grp, cB = 2 * grp + 1;
+    offA = (corner_x(cA) * T::UBY + corner_y(cA)) * T::UBZ + corner_z(cA);
+    offB = (corner_x(cB) * T::UBY + corner_y(cB)) * T::UBZ + corner_z(cB);
+    ubase = w * T::UBY * T::UBZ + je;
+    // lane (g, je) of the pair (py-1, py) writes x-line w-1+cx
+    const int cx = mfma_out_cx(grp), ti = w - 1 + cx;
+    const bool wr = je >= 1 && (unsigned)ti < (unsigned)T::MTX;
+    wr_lo = wr && lane < 32;
+    wr_hi = wr && lane >= 32;
+    sbase = ((ti * T::MTY) * 2 + (1 - cx)) * T::SZ * 3 + (je - 1) * 3;
   }
 
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int je = lane & 15;     // element within the pencil (B and D column)
-  const int grp = lane >> 4;    // B: input corners 2grp, 2grp+1 ; D: output group
+  // Relative displacements of pencil py's elements at this lane's two corners.
+  __device__ __forceinline__ floatx8 rel_disp(const float4* su, int py) const {
+    const float4 u0 = su[ubase + py * T::UBZ];
+    const float4 ua = su[ubase + py * T::UBZ + offA];
+    const float4 ub = su[ubase + py * T::UBZ + offB];
+    const floatx8 f = {ua.x - u0.x, ua.y - u0.y, ua.z - u0.z, 0.f, ub.x - u0.x, ub.y - u0.y, ub.z - u0.z, 0.f};
+    return f;
+  }
 
-  // Element moduli of this wavefront's pencils.
-  float ev[PY];
-  {
-    const int ci = i0 - 1 + w, ck = k0 - 1 + je;
-    const bool okx = (unsigned)ci < (unsigned)g.nx && (unsigned)ck < (unsigned)g.nz;
-    const uint32_t ebase = (uint32_t)(((ci - is) * g.ny + (j0 - 1)) * g.nz + ck);
+  // Pair (py-1, py) complete: write the y-line py-1 share of this column.
+  __device__ __forceinline__ void pair_store(float* slot, int py, const float (&rp)[3], const float (&rc)[3]) const {
+    // y-line py-1: lower lane half when py-1 is even, upper when odd
+    const bool on = ((py - 1) & 1) ? wr_hi : wr_lo;
+    if (on) {
+      float* s = &slot[sbase + (py - 1) * 2 * T::SZ * 3];
 #pragma unroll
-    for (int py = 0; py < PY; ++py) {
-      const bool ok = okx && (unsigned)(j0 - 1 + py) < (unsigned)g.ny;
+      for (int v = 0; v < 3; ++v) s[v] = rp[v] + rc[v];
+    }
+  }
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void mma2(const bf16x8 (&ah)[2], const bf16x8 (&al)[2], const bf16x8& bh,
+                                     const bf16x8& bl, floatx4& d0, floatx4& d1) {
+  d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bh, d0, 0, 0, 0);
+  d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1], bh, d1, 0, 0, 0);
+  if constexpr (SPLIT) {
+    d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bl, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1], bl, d1, 0, 0, 0);
+    d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], bh, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[1], bh, d1, 0, 0, 0);
+  }
+}
+
+// ---------------------------------------------------------------- level 0
+template <bool SPLIT, int MODE>
+__global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
+                                                         const float* __restrict__ E, const float* __restrict__ u,
+                                                         Ep32 ep, Workspace ws) {
+  using T = Tile0;
+  using Mode = EpMode<MODE>;
+  __shared__ float4 su[T::NB];
+  // [x line][y line][side][z][3]: side 0 is the share of the element column
+  // on the -x side of the line, side 1 the +x side.
+  __shared__ float slot[T::SLOT];
+
+  Block b = block_setup<T>(g, ts, mask, u, ep);
+  const long long eslab = (long long)b.is * g.ny * g.nz, ne = (long long)g.nx * g.ny * g.nz;
+  const rsrc_t rs_e = make_rsrc(E + eslab, 4ull * (ne - eslab));
+
+  uint32_t bw[T::NIT][3], bm[T::NIT];
+  brick_load<T>(b, bw, bm);
+  uint32_t ob[T::OIT], om[T::OIT];
+  typename Mode::In oin[T::OIT];
+  out_prefetch<T, MODE>(b, ep, ob, om, oin);
+
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const PencilLane<T> pl(w);
+  // Element moduli of this wavefront's pencils.
+  float ev[T::PY];
+  {
+    const int ci = b.i0 - 1 + w, ck = b.k0 - 1 + pl.je;
+    const bool okx = (unsigned)ci < (unsigned)g.nx && (unsigned)ck < (unsigned)g.nz;
+    const uint32_t ebase = (uint32_t)(((ci - b.is) * g.ny + (b.j0 - 1)) * g.nz + ck);
+#pragma unroll
+    for (int py = 0; py < T::PY; ++py) {
+      const bool ok = okx && (unsigned)(b.j0 - 1 + py) < (unsigned)g.ny;
       ev[py] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                              rs_e, ok ? 4u * (ebase + py * g.nz) : kOOB, 0, 0));
     }
@@ -199,84 +335,147 @@ __global__ void __launch_bounds__(NTHR) k_hex_mfma(Grid g, TileShape ts, const u
   for (int ly = 0; ly < 2; ++ly)
 #pragma unroll
     for (int tl = 0; tl < 2; ++tl) {
-      ah[ly][tl] = load_a(ly, 0, tl, lane);
-      al[ly][tl] = SPLIT ? load_a(ly, 1, tl, lane) : ah[ly][tl];
+      ah[ly][tl] = load_ke(ly, 0, tl, pl.lane);
+      al[ly][tl] = SPLIT ? load_ke(ly, 1, tl, pl.lane) : ah[ly][tl];
     }
-
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int idx = threadIdx.x + it * NTHR;
-    if (idx < NB)
-      su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
-                            (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
-                            (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
-  }
+  brick_store<T>(su, bw, bm);
   __syncthreads();
-
-  // ---- pencils of column w
-  const int cA = 2 * grp, cB = 2 * grp + 1;
-  const int offA = (corner_x(cA) * UBY + corner_y(cA)) * UBZ + corner_z(cA);
-  const int offB = (corner_x(cB) * UBY + corner_y(cB)) * UBZ + corner_z(cB);
-  const int ubase = w * UBY * UBZ + je;
-  // slot writes: lane (g, je) of the pair (py-1, py) writes x-line w-1+cx
-  const int cx = mfma_out_cx(grp), ti = w - 1 + cx;
-  const bool wr = je >= 1 && (unsigned)ti < (unsigned)MTX;
-  const bool wr_lo = wr && lane < 32, wr_hi = wr && lane >= 32;
-  const int sbase = ((ti * MTY) * 2 + (1 - cx)) * SZ * 3 + (je - 1) * 3;   // used only when wr
 
   float rp[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-  for (int py = 0; py < PY; ++py) {
+  for (int py = 0; py < T::PY; ++py) {
     const int ly = py & 1;
-    const float4 u0 = su[ubase + py * UBZ];
-    const float4 ua = su[ubase + py * UBZ + offA];
-    const float4 ub = su[ubase + py * UBZ + offB];
-    const floatx8 f = {ua.x - u0.x, ua.y - u0.y, ua.z - u0.z, 0.f, ub.x - u0.x, ub.y - u0.y, ub.z - u0.z, 0.f};
+    const floatx8 f = pl.rel_disp(su, py);
     const bf16x8 bh = __builtin_convertvector(f, bf16x8);
+    bf16x8 bl = bh;
+    if constexpr (SPLIT) bl = __builtin_convertvector(f - __builtin_convertvector(bh, floatx8), bf16x8);
     floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
-    d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ly][0], bh, d0, 0, 0, 0);
-    d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ly][1], bh, d1, 0, 0, 0);
-    if constexpr (SPLIT) {
-      const bf16x8 bl = __builtin_convertvector(f - __builtin_convertvector(bh, floatx8), bf16x8);
-      d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ly][0], bl, d0, 0, 0, 0);
-      d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ly][1], bl, d1, 0, 0, 0);
-      d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ly][0], bh, d0, 0, 0, 0);
-      d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ly][1], bh, d1, 0, 0, 0);
-    }
+    mma2<SPLIT>(ah[ly], al[ly], bh, bl, d0, d1);
     const float e = ev[py];
     float rc[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) rc[v] = e * d0[v] + row_shift_up(e * d1[v]);
-    if (py > 0) {
-      // y-line py-1: lower lane half when py-1 is even, upper when odd
-      const bool on = ((py - 1) & 1) ? wr_hi : wr_lo;
-      if (on) {
-        float* s = &slot[sbase + (py - 1) * 2 * SZ * 3];
-#pragma unroll
-        for (int v = 0; v < 3; ++v) s[v] = rp[v] + rc[v];
-      }
-    }
+    if (py > 0) pl.pair_store(slot, py, rp, rc);
 #pragma unroll
     for (int v = 0; v < 3; ++v) rp[v] = rc[v];
   }
   __syncthreads();
 
-  // ---- output: sum the two column shares, apply the epilogue
-  double part = 0.0;
-#pragma unroll
-  for (int it = 0; it < OIT; ++it) {
-    int idx = threadIdx.x + it * NTHR;
-    idx = idx < NOUT ? idx : NOUT - 1;   // surplus threads read a valid slot; their stores are dropped
-    const int z = idx % MTZ, line = idx / MTZ, tli = line / MTY, tlj = line % MTY;
-    const float* s0 = &slot[(line * 2) * SZ * 3 + z * 3];
-    float au[3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) au[v] = s0[v] + s0[SZ * 3 + v];
-    const float4 uo = su[((tli + 1) * UBY + tlj + 1) * UBZ + z + 1];
-    const float own[3] = {uo.x, uo.y, uo.z};
-    acc.b = ob[it];
-    part += Mode::store(ep, acc, om[it], au, own, oin[it]);
+  const double part = out_store<T, MODE>(b, ep, slot, su, ob, om, oin);
+  if constexpr (Mode::kReduces) {
+    double v[1] = {part};
+    block_partials<1>(v, ws);
   }
+}
+
+// ---------------------------------------------------------------- level 1
+// Coarse element (I,J,K) of grid g has matrix sum_s E_f(2I+sx, 2J+sy, 2K+sz)
+// Q_s (s = 4 sx + 2 sy + sz) with E_f on the fine grid 2g. Per pencil the
+// eight products Q_s U share the B operand U; each is scaled by its child's
+// modulus per element (column) and accumulated in FP32. The child matrices
+// are streamed per s (8 fragments per lane) and applied to half of the
+// wavefront's pencils at a time, which bounds the registers held.
+template <bool SPLIT, int MODE>
+__global__ void __launch_bounds__(Tile1::NTHR) k_q8_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
+                                                        const float* __restrict__ Ef, const float* __restrict__ u,
+                                                        Ep32 ep, Workspace ws) {
+  using T = Tile1;
+  using Mode = EpMode<MODE>;
+  constexpr int H0 = (T::PY + 1) / 2;   // pencils of the first half
+  __shared__ float4 su[T::NB];
+  __shared__ float slot[T::SLOT];
+
+  Block b = block_setup<T>(g, ts, mask, u, ep);
+  const int fny = 2 * g.ny, fnz = 2 * g.nz;
+  const long long fslab = 2ll * b.is * fny * fnz, fne = 8ll * g.nx * g.ny * g.nz;
+  const rsrc_t rs_e = make_rsrc(Ef + fslab, 4ull * (fne - fslab));
+
+  uint32_t bw[T::NIT][3], bm[T::NIT];
+  brick_load<T>(b, bw, bm);
+  uint32_t ob[T::OIT], om[T::OIT];
+  typename Mode::In oin[T::OIT];
+  out_prefetch<T, MODE>(b, ep, ob, om, oin);
+
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const PencilLane<T> pl(w);
+  // fine offset of child (0,0,0) of the lane's element in pencil py = 0
+  const int ci = b.i0 - 1 + w, ck = b.k0 - 1 + pl.je;
+  const bool okx = (unsigned)ci < (unsigned)g.nx && (unsigned)ck < (unsigned)g.nz;
+  const uint32_t fbase = (uint32_t)(((2 * (ci - b.is)) * fny + 2 * (b.j0 - 1)) * fnz + 2 * ck);
+  auto child_e = [&](int s, int py) -> float {
+    const bool ok = okx && (unsigned)(b.j0 - 1 + py) < (unsigned)g.ny;
+    const uint32_t off = fbase + (uint32_t)(((s >> 2) * fny + 2 * py + ((s >> 1) & 1)) * fnz + (s & 1));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_e, ok ? 4u * off : kOOB, 0, 0));
+  };
+  brick_store<T>(su, bw, bm);
+  __syncthreads();
+
+  float rp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    constexpr int NPH = H0;   // slots per half (the second half uses PY - H0 of them)
+    const int p0 = half ? H0 : 0, np = half ? T::PY - H0 : H0;
+    bf16x8 bh[NPH], bl[NPH];
+    floatx4 a0[NPH], a1[NPH];
+#pragma unroll
+    for (int q = 0; q < NPH; ++q) {
+      if (q < np) {
+        const floatx8 f = pl.rel_disp(su, p0 + q);
+        bh[q] = __builtin_convertvector(f, bf16x8);
+        bl[q] = bh[q];
+        if constexpr (SPLIT) bl[q] = __builtin_convertvector(f - __builtin_convertvector(bh[q], floatx8), bf16x8);
+      }
+      a0[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+      a1[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    float ec[NPH];
+#pragma unroll
+    for (int q = 0; q < NPH; ++q) ec[q] = q < np ? child_e(0, p0 + q) : 0.f;
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {   // not unrolled: one child's operands live at a time
+      bf16x8 ah[2][2], al[2][2];
+#pragma unroll
+      for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+        for (int tl = 0; tl < 2; ++tl) {
+          ah[ly][tl] = load_q8(s, ly, 0, tl, pl.lane);
+          al[ly][tl] = SPLIT ? load_q8(s, ly, 1, tl, pl.lane) : ah[ly][tl];
+        }
+      float en[NPH];
+#pragma unroll
+      for (int q = 0; q < NPH; ++q) en[q] = (s < 7 && q < np) ? child_e(s + 1, p0 + q) : 0.f;
+#pragma unroll
+      for (int q = 0; q < NPH; ++q) {
+        if (q < np) {
+          const int ly = (p0 + q) & 1;
+          floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+          mma2<SPLIT>(ah[ly], al[ly], bh[q], bl[q], d0, d1);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            a0[q][v] = fmaf(ec[q], d0[v], a0[q][v]);
+            a1[q][v] = fmaf(ec[q], d1[v], a1[q][v]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NPH; ++q) ec[q] = en[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NPH; ++q) {
+      if (q < np) {
+        const int py = p0 + q;
+        float rc[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) rc[v] = a0[q][v] + row_shift_up(a1[q][v]);
+        if (py > 0) pl.pair_store(slot, py, rp, rc);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) rp[v] = rc[v];
+      }
+    }
+  }
+  __syncthreads();
+
+  const double part = out_store<T, MODE>(b, ep, slot, su, ob, om, oin);
   if constexpr (Mode::kReduces) {
     double v[1] = {part};
     block_partials<1>(v, ws);
@@ -293,6 +492,17 @@ __global__ void __launch_bounds__(64) k_mfma_probe(const uint4* __restrict__ a, 
   d[lane] = make_float4(r[0], r[1], r[2], r[3]);
 }
 
+template <class T>
+int tile_launch_shape(const Grid& g, TileShape& ts, long long& nb) {
+  ts = TileShape{(g.NZ() + T::MTZ - 1) / T::MTZ, (g.NY() + T::MTY - 1) / T::MTY};
+  nb = (long long)((g.NX() + T::MTX - 1) / T::MTX) * ts.nty * ts.ntz;
+  TG_REQUIRE(nb <= kMaxPartials, "grid too large for the reduction workspace");
+  // every slab offset must stay below 2^31 bytes
+  TG_REQUIRE((long long)(T::MTX + 2) * g.NY() * g.NZ() * 12 < (long long)kMaxRange,
+             "grid cross-section %d x %d nodes too large for 32-bit slab offsets", g.NY(), g.NZ());
+  return TG_OK;
+}
+
 }  // namespace
 
 int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s) {
@@ -304,19 +514,37 @@ int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_
 
 int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const float* u, const Ep32& ep,
                     double* dot, bool split, hipStream_t s) {
-  const TileShape ts{(g.NZ() + MTZ - 1) / MTZ, (g.NY() + MTY - 1) / MTY};
-  const long long nb = (long long)((g.NX() + MTX - 1) / MTX) * ts.nty * ts.ntz;
-  TG_REQUIRE(nb <= kMaxPartials, "grid too large for the reduction workspace");
-  // every slab offset must stay below 2^31 bytes
-  TG_REQUIRE((long long)(MTX + 2) * g.NY() * g.NZ() * 12 < (long long)kMaxRange,
-             "grid cross-section %d x %d nodes too large for 32-bit slab offsets", g.NY(), g.NZ());
+  TileShape ts;
+  long long nb;
+  if (int rc = tile_launch_shape<Tile0>(g, ts, nb)) return rc;
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
     if (split)
-      hipLaunchKernelGGL((k_hex_mfma<true, M>), dim3((unsigned)nb), dim3(NTHR), 0, s, g, ts, mask, E, u, ep,
+      hipLaunchKernelGGL((k_hex_mfma<true, M>), dim3((unsigned)nb), dim3(Tile0::NTHR), 0, s, g, ts, mask, E, u,
+                         ep, workspace());
+    else
+      hipLaunchKernelGGL((k_hex_mfma<false, M>), dim3((unsigned)nb), dim3(Tile0::NTHR), 0, s, g, ts, mask, E, u,
+                         ep, workspace());
+    TG_LAUNCH_CHECK();
+    return EpMode<M>::kReduces ? finalize_sums(1, (int)nb, dot, nullptr, s) : TG_OK;
+  });
+}
+
+int launch_q8_mfma(const Grid& g, const uint8_t* mask, const float* Ef, const float* u, const Ep32& ep, double* dot,
+                   bool split, hipStream_t s) {
+  TileShape ts;
+  long long nb;
+  if (int rc = tile_launch_shape<Tile1>(g, ts, nb)) return rc;
+  // fine-grid slab offsets: NW element columns = 2 NW fine x-planes
+  TG_REQUIRE((long long)(2 * Tile1::NW) * (2 * g.ny) * (2 * g.nz) * 4 < (long long)kMaxRange,
+             "grid cross-section too large for 32-bit slab offsets");
+  return with_mode(ep, [&](const auto& em) {
+    constexpr int M = std::decay_t<decltype(em)>::kMode;
+    if (split)
+      hipLaunchKernelGGL((k_q8_mfma<true, M>), dim3((unsigned)nb), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef, u, ep,
                          workspace());
     else
-      hipLaunchKernelGGL((k_hex_mfma<false, M>), dim3((unsigned)nb), dim3(NTHR), 0, s, g, ts, mask, E, u, ep,
+      hipLaunchKernelGGL((k_q8_mfma<false, M>), dim3((unsigned)nb), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef, u, ep,
                          workspace());
     TG_LAUNCH_CHECK();
     return EpMode<M>::kReduces ? finalize_sums(1, (int)nb, dot, nullptr, s) : TG_OK;

@@ -23,6 +23,8 @@ int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const fl
 int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s);
 int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
                    double* dot, hipStream_t s);
+int launch_q8_mfma(const Grid& g, const uint8_t* mask, const float* Ef, const float* u, const Ep32& ep, double* dot,
+                   bool split, hipStream_t s);
 
 namespace {
 
@@ -331,6 +333,10 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
         return E::reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
       });
     }
+    case TG_OP_Q8_BF16:
+    case TG_OP_Q8_BF16X3:
+      TG_REQUIRE(op->weights, "weights NULL");
+      return launch_q8_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_Q8_BF16X3, s);
     case TG_OP_STENCIL_F32:
       TG_REQUIRE(op->stencil, "stencil NULL");
       return launch_stencil(g, op->mask, op->stencil, u, e, ep->dot, s);

@@ -42,7 +42,8 @@ class _Level:
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
-                L.TG_OP_STORED_F32: "stored_f32", L.TG_OP_STENCIL_F32: "stencil_f32"}
+                L.TG_OP_STORED_F32: "stored_f32", L.TG_OP_STENCIL_F32: "stencil_f32",
+                L.TG_OP_Q8_BF16: "galerkin_q8_bf16_mfma", L.TG_OP_Q8_BF16X3: "galerkin_q8_bf16x3_mfma"}
 
 _EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
                    L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8}
@@ -66,7 +67,8 @@ def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: 
 
 class GMG:
     def __init__(self, dims, mask0: np.ndarray, *, n_levels: int = 4, fine_op: int = L.TG_OP_HEX_BF16,
-                 fine_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
+                 fine_accurate_op: int | None = None, l1_q8_op: int = L.TG_OP_Q8_F32,
+                 l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8"):
         if n_levels < 2:
@@ -115,17 +117,19 @@ class GMG:
         l1 = self.levels[1]
         if level1 not in ("q8", "stencil"):
             raise ValueError(f"level1 must be 'q8' or 'stencil'; got {level1!r}")
-        if level1 == "q8" and n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
-            l1.kind = L.TG_OP_Q8_F32
         for lv in range(1, n_levels):
             self.levels[lv].kind_accurate = self.levels[lv].kind
+        if level1 == "q8" and n_levels > 2 and all(n % 2 == 0 for n in self.dims[0]) and l1.consistent:
+            l1.kind = l1_q8_op
+            l1.kind_accurate = l1_q8_accurate_op if l1_q8_accurate_op is not None else l1_q8_op
+        self.l1_q8 = l1.kind in (L.TG_OP_Q8_F32, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16X3)
         # Galerkin build plan: aligned levels come straight from the fine E by
         # one gather + DGEMM (levels 1 and 2); the rest from the level above.
         even = lambda d: all(n % 2 == 0 for n in d)   # noqa: E731
         self.l1_from_fine = even(self.dims[0]) and l1.consistent
         self.l2_from_fine = (n_levels > 2 and self.l1_from_fine and even(self.dims[1])
                              and self.levels[2].consistent)
-        self.need_l1_mats = l1.kind != L.TG_OP_Q8_F32 or n_levels == 2 or not self.l2_from_fine
+        self.need_l1_mats = not self.l1_q8 or n_levels == 2 or not self.l2_from_fine
         work = 0
         if self.l1_from_fine and self.need_l1_mats:
             work = max(work, 8 * self.levels[1].n_elem)
@@ -189,7 +193,7 @@ class GMG:
                            L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64),
                            L.ptr(c.mats64), m32, stream)
             if lv < last_lv:
-                if c.kind == L.TG_OP_Q8_F32:
+                if lv == 1 and self.l1_q8:
                     L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
                 else:
                     if c.stencil is None:

@@ -31,18 +31,21 @@ from .. import fem
 from . import lib as L
 from .gmg import GMG, launch_op
 
-# precision -> (Krylov operator, level-0 smoother operator)
+# precision -> (Krylov operator, level-0 smoother operator,
+#               level-1 accurate operator, level-1 smoother operator)
 #   bf16       : split BF16 MFMA (hi/lo operands, 3 passes) for the PCG
-#                product, single-pass BF16 MFMA in the smoother
+#                product and the post-correction products, single-pass BF16
+#                MFMA in the smoothers (levels 0 and 1)
 #   bf16-plain : single-pass BF16 MFMA everywhere (the inner solve then
 #                stalls near the BF16 rounding level; kept for study)
 #   bf16x3     : split BF16 MFMA everywhere
 #   fp32       : FP32 vector FMA everywhere (the plain FP32 path)
+# The level-1 entries apply when level 1 is the implicit Galerkin (Q8) form.
 PRECISIONS = {
-    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16),
-    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16),
-    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3),
-    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32),
+    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16),
+    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
+    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16X3),
+    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_Q8_F32, L.TG_OP_Q8_F32),
 }
 
 
@@ -87,8 +90,9 @@ class HipStateSolver:
         self.tol, self.max_krylov = float(tol), int(max_krylov)
         self.inner_rtol, self.max_restarts = float(inner_rtol), int(max_restarts)
         self.warm_start = warm_start
-        self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=PRECISIONS[precision][1],
-                       fine_accurate_op=PRECISIONS[precision][0], fine_degree=fine_degree,
+        ops = PRECISIONS[precision]
+        self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[0],
+                       l1_q8_op=ops[3], l1_q8_accurate_op=ops[2], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
                        level1=level1, device=self.dev)
         f32 = dict(dtype=torch.float32, device=self.dev)
