@@ -3,17 +3,22 @@
 
 Headline: SIMP iterations/s on the 160x80x80 hex8 cantilever (1,024,000
 elements, 3.17M DOF; x=0 face clamped, total -1 z line load on the x=160,
-z=0 edge; volfrac 0.3, rmin 1.5, move 0.2; 4-level Galerkin GMG with
-Chebyshev degree 3; PCG tol 1e-5 relative residual, max_krylov 800), for the
-BF16-MFMA path and the plain FP32 path. A step is one full SIMP iteration:
-filter, projection, moduli, GMG setup, state solve to tol (FP64 true
-residual), compliance, sensitivity, filter transpose, OC bisection. Inputs
-are resident in HBM; the density starts at volfrac, W warm-up iterations are
-run untimed, then K iterations are timed between device synchronisations.
+z=0 edge; volfrac 0.3, rmin 1.5, move 0.2; Galerkin GMG-PCG with Chebyshev
+degree 3 on the fine level; PCG tol 1e-5 relative residual, max_krylov 800),
+for the BF16-MFMA path and the plain FP32 path. A step is one full SIMP
+iteration: filter, projection, moduli, GMG setup, state solve to tol (FP64
+true residual), compliance, sensitivity, filter transpose, OC bisection.
+Inputs are resident in HBM; the density starts at volfrac, W warm-up
+iterations are run untimed, then K iterations are timed between device
+synchronisations (defaults 2 + 48: the workload's 50 OC iterations). The
+hierarchy depth is the solver's default (coarsened until the coarsest level
+fits the one-workgroup dense inverse); the 4-level hierarchy of the workload
+description is reported as a variant.
 
 Also reported: matvec GB/s on the 128x64x64 grid (524,288 elements,
-algorithmic bytes 8*ndof + 4*n_elem per product), the live roofline of the
-dominant kernel, and a CPU baseline from the FP64 oracle on the host cores.
+algorithmic bytes 8*ndof + 4*n_elem per product), the roofline of the
+dominant level-0 kernel, and a CPU baseline from the FP64 oracle on the host
+cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -29,17 +34,23 @@ import time
 import numpy as np
 
 METRIC = "SIMP iter/s @1M hex8 (BF16-MFMA GMG vs FP32 plain, rtol 1e-5); matvec GB/s"
-C2 = dict(dims=(160, 80, 80), volfrac=0.30, rmin=1.5, move=0.2, n_levels=4, fine_degree=3, coarse_degree=2,
+# The 1M-element cantilever of the headline. n_levels "auto" is the solver's
+# default: coarsen until the coarsest level fits one workgroup's dense inverse.
+C2 = dict(dims=(160, 80, 80), volfrac=0.30, rmin=1.5, move=0.2, n_levels="auto", fine_degree=3, coarse_degree=2,
           tol=1e-5, max_krylov=800)
 C1 = (128, 64, 64)
 HBM_PEAK_GBPS = 8000.0
+EP_NAMES = {0: "STORE", 1: "DOT", 2: "RESID", 3: "RESID_NORM", 4: "CHEB", 5: "CHEB_R", 6: "RESID_UPDATE"}
 
 
 def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool, n_levels=None):
+def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated: bool = False):
+    """Run warmup + steps SIMP iterations from rho = volfrac; time the steps.
+    Level-0 operator launches are counted per (kernel, epilogue) in the timed
+    region (no events: nothing but a counter is added to the loop)."""
     import torch
 
     import topogpu_amd as tg
@@ -56,8 +67,8 @@ def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool, n_l
     for i in range(warmup):
         r = eng.step()
         _log(f"{precision} warmup {i + 1}/{warmup}: C={r.compliance:.6g} its={r.iters} conv={r.converged}")
-    timer = KernelTimer() if timed_kernels else None
-    eng.fea.gmg.timer = timer
+    counter = KernelTimer(events=False)
+    eng.fea.gmg.timer = counter
     torch.cuda.synchronize()
     rows = []
     t0 = time.perf_counter()
@@ -68,31 +79,34 @@ def simp_bench(precision: str, steps: int, warmup: int, timed_kernels: bool, n_l
     dt = time.perf_counter() - t0
     eng.fea.gmg.timer = None
     for i, r in enumerate(rows):
-        _log(f"{precision} step {i + 1}/{steps}: C={r.compliance:.6g} its={r.iters} restarts={r.restarts} "
-             f"relres={r.rel_residual:.2e} oc={r.oc_steps}")
+        if i < 3 or i >= steps - 3 or (i + 1) % 10 == 0:
+            _log(f"{precision} step {i + 1}/{steps}: C={r.compliance:.6g} its={r.iters} restarts={r.restarts} "
+                 f"relres={r.rel_residual:.2e} oc={r.oc_steps}")
     stats = {
         "iter_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "n_levels": len(eng.fea.gmg.levels),
         "coarsest_dofs": eng.fea.gmg.n_coarse,
         "pcg_iters_mean": float(np.mean([r.iters for r in rows])),
+        "pcg_iters_max": int(max(r.iters for r in rows)),
         "ir_restarts_mean": float(np.mean([r.restarts for r in rows])),
         "max_rel_residual": float(max(r.rel_residual for r in rows)),
         "all_converged": bool(all(r.converged for r in rows)),
         "compliance_last": float(rows[-1].compliance),
     }
-    kernels = timer.summary() if timer else {}
-    if timed_kernels:
-        for name, rec in saturated_kernel_times(eng, steps).items():
-            kernels.setdefault(name, {}).update(rec)
+    launches = {f"{name}/{EP_NAMES[mode]}": n / steps for (name, mode), n in counter.counts.items()}
+    kernels = {}
+    if saturated:
+        kernels = saturated_kernel_times(eng, counter)
     n = {"ndof": eng.fea.n, "n_elem": eng.fea.n_elem}
     del eng
     torch.cuda.empty_cache()
-    return stats, kernels, n
+    return stats, launches, kernels, n
 
 
-def saturated_kernel_times(eng, steps: int, reps: int = 50) -> dict:
-    """Average duration of the level-0 kernels on the live operands (E,
-    masks, dinv of the last SIMP iteration), launched back to back between
-    two events so host launch gaps do not enter the measurement."""
+def saturated_kernel_times(eng, counter, reps: int = 50) -> dict:
+    """Average duration of every level-0 (kernel, epilogue) pair the timed
+    region launched, on the live operands of the last SIMP iteration (E,
+    masks, dinv), launched back to back between two HIP events on the
+    kernels' stream so host launch gaps do not enter the measurement."""
     import torch
 
     from topogpu_amd.hip import lib as L
@@ -106,13 +120,20 @@ def saturated_kernel_times(eng, steps: int, reps: int = 50) -> dict:
     r, x, d = lv0.r.clone(), lv0.x.clone(), torch.zeros_like(lv0.da)
     y = torch.zeros_like(lv0.da)
     dot = torch.zeros(1, dtype=torch.float64, device=u.device)
-    cases = {
-        KERNEL_NAMES[g.levels[0].kind]: (g.op(0), L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1),
-                                         L.TG_EP_CHEB_R, None),
-        KERNEL_NAMES[eng.fea.fine_op().kind]: (eng.fea.fine_op(), L.ep(L.TG_EP_DOT, dot=dot), L.TG_EP_DOT, y),
+    ops = {KERNEL_NAMES[k]: (g.op(0, accurate=acc)) for acc in (False, True) for k in [g.op(0, accurate=acc).kind]}
+    ops[KERNEL_NAMES[eng.fea.fine_op().kind]] = eng.fea.fine_op()
+    eps = {
+        L.TG_EP_CHEB_R: (L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1), None),
+        L.TG_EP_RESID_UPDATE: (L.ep(L.TG_EP_RESID_UPDATE, r=r), None),
+        L.TG_EP_DOT: (L.ep(L.TG_EP_DOT, dot=dot), y),
+        L.TG_EP_STORE: (L.ep(L.TG_EP_STORE), y),
     }
     out = {}
-    for name, (op, ep, mode, yy) in cases.items():
+    for (name, mode), cnt in counter.counts.items():
+        if name not in ops or mode not in eps:
+            continue
+        op = ops[name]
+        ep, yy = eps[mode]
         for _ in range(3):
             L.call("tg_apply32", op, L.ptr(u), L.ptr(yy), ep, st)
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -123,9 +144,46 @@ def saturated_kernel_times(eng, steps: int, reps: int = 50) -> dict:
         torch.cuda.synchronize()
         us = 1e3 * s.elapsed_time(e) / reps
         nb = hex_bytes(n, ne, mode)
-        out[name] = {"saturated_avg_us": us, "saturated_epilogue": {L.TG_EP_CHEB_R: "CHEB_R", L.TG_EP_DOT: "DOT"}[mode],
-                     "saturated_bytes_per_launch": nb, "saturated_gbps": nb / (us * 1e-6) / 1e9}
+        out[f"{name}/{EP_NAMES[mode]}"] = {"avg_us": us, "bytes_per_launch": nb, "gbps": nb / (us * 1e-6) / 1e9,
+                                          "launches_per_step": None}
     return out
+
+
+def kernel_probe(precision: str, reps: int = 20) -> dict:
+    """Reference copy of known size, then the dominant level-0 kernel on live
+    operands: the pair a counter run scales its byte counts with."""
+    import torch
+
+    import topogpu_amd as tg
+    from topogpu_amd.bc import generate_bc
+    from topogpu_amd.hip import lib as L
+    from topogpu_amd.hip.gmg import hex_bytes
+    from topogpu_amd.hip.simp import HipSIMP
+
+    prob = tg.unit_cantilever(C2["dims"], volfrac=C2["volfrac"], rmin=C2["rmin"], load="line")
+    bc = generate_bc(prob.spec)
+    eng = HipSIMP(C2["dims"], bc.free_dofs, prob.force_override, volfrac=C2["volfrac"], rmin=C2["rmin"],
+                  move=C2["move"], precision=precision, n_levels=C2["n_levels"], fine_degree=C2["fine_degree"],
+                  coarse_degree=C2["coarse_degree"], tol=C2["tol"], max_krylov=C2["max_krylov"])
+    for _ in range(2):
+        eng.step()
+    copy_bytes = 256 << 20
+    src = torch.ones(copy_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+    L.call("tg_copy32", src.numel(), L.ptr(dst), L.ptr(src), st)   # k_copy32: reads and writes copy_bytes
+    g = eng.fea.gmg
+    lv0 = g.levels[0]
+    u, r, x, d = lv0.da.clone(), lv0.r.clone(), lv0.x.clone(), torch.zeros_like(lv0.da)
+    op = g.op(0)
+    ep = L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1)
+    for _ in range(reps):
+        L.call("tg_apply32", op, L.ptr(u), None, ep, st)
+    torch.cuda.synchronize()
+    return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "kernel": "k_hex_mfma<false, CHEB_R>",
+            "launches": reps,
+            "algorithmic_bytes_per_launch": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_R)}
 
 
 def matvec_bench(reps: int = 200, warmup: int = 5):
@@ -243,15 +301,19 @@ def cpu_baseline(fine_ops_per_iter: float, sample_matvecs: int = 2):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=48, help="timed SIMP iterations (default: with the warm-up, "
+                                                           "the 50 OC iterations of the workload)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="bf16")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-matvec", action="store_true")
-    ap.add_argument("--no-variants", action="store_true", help="skip the auto-depth hierarchy runs")
+    ap.add_argument("--no-variants", action="store_true", help="skip the fixed 4-level hierarchy run")
     ap.add_argument("--levels", default=None, help="override the hierarchy depth of the headline run (int or auto)")
     ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="two warm-up SIMP iterations, then a 256 MiB device copy and 20 launches of the level-0 "
+                         "BF16 CHEB_R operator on the live operands (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode (see DESIGN.md for how it would "
@@ -265,47 +327,52 @@ def main() -> int:
     if args.matvec_only:
         print(json.dumps({"matvec": matvec_bench()}), flush=True)
         return 0
+    if args.kernel_probe:
+        print(json.dumps({"kernel_probe": kernel_probe(args.precision)}), flush=True)
+        return 0
 
     levels = None if args.levels is None else (args.levels if args.levels == "auto" else int(args.levels))
-    bf, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, timed_kernels=True, n_levels=levels)
-    _log(f"{args.precision}: {bf['iter_per_s']:.3f} iter/s")
+    bf, launches, kernels, sizes = simp_bench(args.precision, args.steps, args.warmup, n_levels=levels,
+                                              saturated=True)
+    _log(f"{args.precision}: {bf['iter_per_s']:.3f} iter/s ({bf['n_levels']} levels)")
     fp = None
     if not args.no_fp32:
-        fp, _, _ = simp_bench("fp32", args.steps, args.warmup, timed_kernels=False)
+        fp, _, _, _ = simp_bench("fp32", args.steps, args.warmup, n_levels=levels)
         _log(f"fp32: {fp['iter_per_s']:.3f} iter/s")
     variants = {}
     if not args.no_variants:
-        # same problem and tolerance, hierarchy depth chosen so the dense
-        # FP64 coarsest level stays small (n_levels="auto")
-        for prec in (args.precision, "fp32"):
-            st, _, _ = simp_bench(prec, args.steps, args.warmup, timed_kernels=False, n_levels="auto")
-            variants[f"{prec}_auto_levels"] = st
-            _log(f"{prec} auto levels ({st['n_levels']}): {st['iter_per_s']:.3f} iter/s")
+        # the hierarchy depth the workload description names (4 levels,
+        # coarsest 7,260 DOFs inverted densely each iteration)
+        st, _, _, _ = simp_bench(args.precision, args.steps, args.warmup, n_levels=4)
+        variants[f"{args.precision}_4_levels"] = st
+        _log(f"{args.precision} 4 levels: {st['iter_per_s']:.3f} iter/s")
     mv = None if args.no_matvec else matvec_bench()
-    # dominant level-0 kernel: most device time per SIMP step (launch count
-    # from the in-loop timer x saturated per-launch duration)
-    timed = {k: v for k, v in kernels.items() if "saturated_avg_us" in v and "launches" in v}
-    dom_name, dom = max(timed.items(), key=lambda kv: kv[1]["saturated_avg_us"] * kv[1]["launches"])
-    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["saturated_gbps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": dom["saturated_gbps"] / HBM_PEAK_GBPS, "traffic": None,
-                "avg_us": dom["saturated_avg_us"], "epilogue": dom["saturated_epilogue"],
-                "launches_per_step": dom["launches"] / args.steps,
-                "algorithmic_bytes_per_launch": dom["saturated_bytes_per_launch"]}
-    fine_ops = sum(k["launches"] for k in kernels.values()) / args.steps
+    # dominant level-0 (kernel, epilogue): most device time per SIMP step
+    for k, v in kernels.items():
+        v["launches_per_step"] = launches.get(k, 0.0)
+    dom_name, dom = max(kernels.items(), key=lambda kv: kv[1]["avg_us"] * kv[1]["launches_per_step"])
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbps"], "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": dom["gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                "avg_us": dom["avg_us"], "launches_per_step": dom["launches_per_step"],
+                "algorithmic_bytes_per_launch": dom["bytes_per_launch"]}
+    fine_ops = sum(launches.values())
     cpu = None if args.no_cpu else cpu_baseline(fine_ops)
     line = {
         "metric": METRIC, "value": bf["iter_per_s"], "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": bf["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16" if args.precision.startswith("bf16") else "fp32",
         "data": "synthetic",
-        "config": {"workload": "SIMP cantilever 160x80x80 hex8 (1,024,000 elem), line load, 4-level Galerkin "
-                               "GMG, Chebyshev degree 3, PCG tol 1e-5",
+        "config": {"workload": "SIMP cantilever 160x80x80 hex8 (1,024,000 elem), x=0 clamped, line load on the "
+                               "x=160 z=0 edge, Galerkin GMG-PCG (Chebyshev degree 3 fine / 2 coarse), tol 1e-5, "
+                               "iterations 1-50 from rho=volfrac",
                    "precision": args.precision, **{k: v for k, v in C2.items() if k != "dims"},
+                   "n_levels_used": bf["n_levels"], "coarsest_dofs": bf["coarsest_dofs"],
                    "grid": "160x80x80", "ndof": sizes["ndof"], "n_elem": sizes["n_elem"]},
-        "solve": {k: bf[k] for k in ("pcg_iters_mean", "ir_restarts_mean", "max_rel_residual", "all_converged",
-                                     "compliance_last")},
+        "solve": {k: bf[k] for k in ("pcg_iters_mean", "pcg_iters_max", "ir_restarts_mean", "max_rel_residual",
+                                     "all_converged", "compliance_last")},
         "fp32_plain": fp, "speedup_vs_fp32": (bf["iter_per_s"] / fp["iter_per_s"]) if fp else None,
-        "variants": variants, "matvec": mv, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "variants": variants, "level0_launches_per_step": launches, "matvec": mv, "kernels": kernels,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -238,6 +238,8 @@ int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, dou
 /* Fill helpers (graph-safe replacements for memset). */
 int tg_fill32(int64_t n, float* x, float value, void* stream);
 int tg_fill64(int64_t n, double* x, double value, void* stream);
+/* dst = src (n floats): p = z of _cupy_pcg (solver_v2.py:603) as a kernel. */
+int tg_copy32(int64_t n, float* dst, const float* src, void* stream);
 
 /* ---- SIMP density / sensitivity / OC (FP64) ----------------------------- */
 

@@ -213,6 +213,28 @@ def test_run_simp_matches_reference_golden(golden, name):
     assert r["final_compliance"] == pytest.approx(float(g["final_compliance"]), rel=1e-9)
 
 
+@pytest.mark.parametrize("name", ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2"])
+def test_generate_bc_matches_reference_golden(golden, name):
+    """Fixed/free DOFs and the load vector equal those the reference's own
+    generate_bc produced for the golden cases (bc_generator.py:381-447)."""
+    import json
+    from types import SimpleNamespace as NS
+
+    from conftest import GOLDEN
+
+    g = golden(name)
+    spec = json.loads((GOLDEN / f"{name}.json").read_text())["case"]["spec"]
+    ns = NS(**{k: v for k, v in spec.items() if k not in ("supports", "loads")},
+            supports=[NS(**d) for d in spec["supports"]], loads=[NS(**d) for d in spec["loads"]])
+    for ld in ns.loads:
+        for k in ("fx", "fy", "fz"):
+            setattr(ld, k, getattr(ld, k, 0.0))
+    fixed, free, F, ndof = generate_bc(ns)
+    assert np.array_equal(free, g["free"])
+    assert np.array_equal(F, g["F"])
+    assert fixed.size + free.size == ndof == F.size
+
+
 @pytest.mark.parametrize("nf,nc", [(8, 4), (7, 3), (3, 1), (1, 1), (2, 1)])
 def test_prolongation_axis_reproduces_linear(nf, nc):
     W = gmg._axis_weights(nf, nc)

@@ -90,6 +90,12 @@ int vec_blocks(long long n) {
   return (int)b;
 }
 
+__global__ void k_copy32(long long n, float* __restrict__ dst, const float* __restrict__ src) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill32(long long n, float* x, float v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -287,6 +293,13 @@ extern "C" int tg_fill32(int64_t n, float* x, float v, void* s) {
   TG_REQUIRE(n >= 0 && (n == 0 || x), "bad fill32 args");
   if (n == 0) return TG_OK;
   hipLaunchKernelGGL(k_fill32, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, v);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+extern "C" int tg_copy32(int64_t n, float* dst, const float* src, void* s) {
+  TG_REQUIRE(n >= 0 && (n == 0 || (dst && src)), "bad copy32 args");
+  if (n == 0) return TG_OK;
+  hipLaunchKernelGGL(k_copy32, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, dst, src);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

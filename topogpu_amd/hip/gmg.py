@@ -62,7 +62,7 @@ def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: 
         L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream)
         return
     timer.launch(KERNEL_NAMES[op.kind], hex_bytes(n_dof, n_elem, ep.mode),
-                 lambda: L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream))
+                 lambda: L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream), ep.mode)
 
 
 class GMG:

@@ -60,7 +60,7 @@ class SolveStats:
 class HipStateSolver:
     """Device state solver for one structured grid and one load case."""
 
-    def __init__(self, dims, free: np.ndarray, F: np.ndarray, *, precision: str = "bf16", n_levels: int = 4,
+    def __init__(self, dims, free: np.ndarray, F: np.ndarray, *, precision: str = "bf16", n_levels="auto",
                  fine_degree: int = 2, coarse_degree: int = 2, cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
@@ -151,7 +151,7 @@ class HipStateSolver:
             z = self.gmg.apply(self.r, st)
             its += 1
             L.call("tg_dot32", self.n, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0:1]), None, st)
-            torch_copy(self.p, z)
+            L.call("tg_copy32", self.n, L.ptr(self.p), L.ptr(z), st)
             while True:
                 ep = L.ep(L.TG_EP_DOT, dot=s[1:2])
                 launch_op(self.gmg.timer, 0, op, L.ptr(self.p), L.ptr(self.q), ep, st, self.n, self.n_elem)
@@ -177,7 +177,3 @@ class HipStateSolver:
         L.call("tg_dot64", self.n, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5:6]), self.stream)
         return float(self.s[5].item())
 
-
-def torch_copy(dst: torch.Tensor, src: torch.Tensor) -> None:
-    """Device-to-device copy (plumbing)."""
-    dst.copy_(src)

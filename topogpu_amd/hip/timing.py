@@ -1,8 +1,11 @@
-"""Optional per-launch timing of selected kernels with HIP events.
+"""Optional accounting of selected kernel launches.
 
-Events are recorded on the stream the kernel is launched on (torch's current
-stream, which is the stream every libtopohip call here receives), so the
-measured durations are those of the kernels themselves.
+``KernelTimer(events=True)`` brackets every launch with HIP events recorded
+on the stream the kernel is launched on (torch's current stream, which every
+libtopohip call here receives), so the measured durations are those of the
+kernels themselves. Events cost host time per launch, so a timed benchmark
+region uses ``events=False``: launches and algorithmic bytes are counted per
+(kernel, epilogue mode) and nothing else is done.
 """
 
 from __future__ import annotations
@@ -13,10 +16,18 @@ import torch
 
 
 class KernelTimer:
-    def __init__(self):
-        self.records = defaultdict(list)   # name -> [(start, end, bytes)]
+    def __init__(self, events: bool = True):
+        self.events = events
+        self.records = defaultdict(list)          # name -> [(start, end, bytes)]
+        self.counts = defaultdict(int)            # (name, mode) -> launches
+        self.bytes = defaultdict(float)           # (name, mode) -> bytes
 
-    def launch(self, name: str, nbytes: float, fn):
+    def launch(self, name: str, nbytes: float, fn, mode: int = -1):
+        self.counts[(name, mode)] += 1
+        self.bytes[(name, mode)] += float(nbytes)
+        if not self.events:
+            fn()
+            return
         s = torch.cuda.Event(enable_timing=True)
         e = torch.cuda.Event(enable_timing=True)
         s.record()
@@ -25,16 +36,23 @@ class KernelTimer:
         self.records[name].append((s, e, float(nbytes)))
 
     def summary(self) -> dict:
-        torch.cuda.synchronize()
+        """Per kernel name: launches, and with events the average duration
+        and the achieved algorithmic bandwidth."""
         out = {}
-        for name, recs in self.records.items():
-            ms = [s.elapsed_time(e) for s, e, _ in recs]
-            tot_ms = sum(ms)
-            tot_b = sum(b for _, _, b in recs)
-            out[name] = {"launches": len(recs), "avg_us": 1e3 * tot_ms / max(len(recs), 1),
-                         "bytes_per_launch": tot_b / max(len(recs), 1),
-                         "gbps": tot_b / max(tot_ms * 1e-3, 1e-30) / 1e9}
+        for (name, _), n in self.counts.items():
+            out.setdefault(name, {"launches": 0})["launches"] += n
+        if self.events:
+            torch.cuda.synchronize()
+            for name, recs in self.records.items():
+                ms = [s.elapsed_time(e) for s, e, _ in recs]
+                tot_ms = sum(ms)
+                tot_b = sum(b for _, _, b in recs)
+                out[name].update({"avg_us": 1e3 * tot_ms / max(len(recs), 1),
+                                  "bytes_per_launch": tot_b / max(len(recs), 1),
+                                  "gbps": tot_b / max(tot_ms * 1e-3, 1e-30) / 1e9})
         return out
 
     def reset(self):
         self.records.clear()
+        self.counts.clear()
+        self.bytes.clear()
