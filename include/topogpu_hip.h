@@ -266,6 +266,14 @@ int tg_sensitivity64(const tg_grid* g, const double* U, const double* rho_phys,
                      const double* dh, double penal, double e_range, double* ce, double* dcf,
                      void* stream);
 
+/* F.u and u^T K u (K = sum_e E_e KE over elements) in double-double
+ * precision: out = {Fu_hi, Fu_lo, uKu_hi, uKu_lo}. u must be zero on fixed
+ * DOFs. 2 F.u - u^T K u equals the compliance F.u* to second order in the
+ * solve error; with the double-double sums its finite differences resolve
+ * per-element gradients (compliance, pub_simp_solver.py:356-360). */
+int tg_compliance_dd(const tg_grid* g, const double* E, const double* u, const double* F, double* out,
+                     void* stream);
+
 /* OC bisection of pub_simp_solver.py:319-340 on the device.
  * oc_prepare: q = rho*sqrt(-min(dc,-1e-12)/max(dv,1e-12)),
  *   lo = rho-move, hi = rho+move ; state <- {l1=0, l2=1e9} and the first

@@ -1,0 +1,101 @@
+"""Adjoint gradient of the compliance (torch.autograd.Function) on the
+64x32x32 adjoint-check case: rho ~ U[0.1, 1] from default_rng(2), filter
+rmin 1.5, x=0 face clamped, unit -z load at node (64, 16, 0).
+
+* against FP64 central finite differences (h = 1e-4) on 64 elements drawn
+  from the same generator (default_rng(2), after rho: the seed is reused, as
+  the case states). The differences are taken on 2 F.u - u.K u, which equals
+  the compliance to second order in the solve error, so the solver tolerance
+  does not enter the difference quotient, with double-double sums and the
+  difference taken component-wise, so FP64 rounding of the O(1) compliance
+  does not enter it either;
+* against the oracle's sensitivity chain (pub_simp_solver.py:361-371: element
+  compliance, SIMP and Heaviside derivatives, transposed filter) on all
+  elements, evaluated on the state the device solve produced.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from topogpu_amd.hip.autograd import HipCompliance
+
+from gpu_util import DEV
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (64, 32, 32)
+
+
+def _c3():
+    nelx, nely, nelz = DIMS
+    from types import SimpleNamespace as NS
+    spec = NS(nelx=nelx, nely=nely, nelz=nelz, Lx=float(nelx), Ly=float(nely), Lz=float(nelz),
+              supports=[NS(edge="left", constraint="fixed")],
+              loads=[NS(x=float(nelx), y=nely / 2.0, z=0.0, fx=0.0, fy=0.0, fz=-1.0)])
+    _, free, F, ndof = oracle.generate_bc(spec)
+    n = nelx * nely * nelz
+    rng = np.random.default_rng(2)
+    rho = rng.uniform(0.1, 1.0, n)
+    elems = rng.choice(n, 64, replace=False)
+    return free, F, ndof, rho, elems
+
+
+@pytest.fixture(scope="module")
+def c3(gpu):
+    free, F, ndof, rho, elems = _c3()
+    comp = HipCompliance(DIMS, free, F, rmin=1.5, precision="bf16", tol=1e-11)
+    r = torch.tensor(rho, dtype=torch.float64, device=DEV, requires_grad=True)
+    C = comp(r)
+    C.backward()
+    return dict(free=free, F=F, ndof=ndof, rho=rho, elems=elems, comp=comp, C=float(C.detach()), grad=r.grad.cpu().numpy(),
+                u=comp.model.fea.u.cpu().numpy())
+
+
+def test_adjoint_matches_central_fd(c3):
+    comp, rho, g = c3["comp"], c3["rho"], c3["grad"]
+    h = 1e-4
+    base = torch.tensor(rho, dtype=torch.float64, device=DEV)
+    c0 = comp.variational(base)
+    assert c0 == pytest.approx(c3["C"], rel=1e-9)
+    errs, resolvable = [], []
+    for k in c3["elems"]:
+        rp, rm = base.clone(), base.clone()
+        rp[k] += h
+        rm[k] -= h
+        fd = comp.central_difference(comp.variational_parts(rp), comp.variational_parts(rm), h)
+        errs.append(abs(fd - g[k]) / abs(g[k]))
+        # the difference quotient resolves g_k when the compliance change
+        # 2h|g_k| is well above the ~1e-16 relative noise of FP64 state solves
+        resolvable.append(2 * h * abs(g[k]) >= 1e-10 * c0)
+    errs, resolvable = np.array(errs), np.array(resolvable)
+    assert resolvable.sum() >= 56
+    assert errs[resolvable].max() < 1e-6, np.sort(errs[resolvable])[-4:]
+    assert np.median(errs) < 2e-7
+    assert errs.max() < 1e-5, np.sort(errs)[-4:]
+
+
+def test_adjoint_matches_oracle_chain_all_elements(c3):
+    rho, g = c3["rho"], c3["grad"]
+    u = c3["u"]                                     # state of the backward solve
+    H = oracle.build_density_filter(*DIMS, 1.5)
+    rf = H @ rho
+    rp = oracle.heaviside(rf, 1.0)
+    dh = oracle.heaviside_deriv(rf, 1.0)
+    ce = oracle.element_compliance(u, oracle.edof_table_3d(*DIMS))
+    dc = H.T @ (-3.0 * (oracle.E0 - oracle.EMIN) * rp ** 2 * ce * dh)
+    assert np.abs(g - dc).max() < 1e-12 * np.abs(dc).max()
+    assert np.all(g < 0.0)                          # more material never raises compliance
+
+
+def test_backward_recomputes_state(c3):
+    """The state is recomputed from zero in backward (not kept), and a
+    second backward gives the same gradient."""
+    comp = c3["comp"]
+    r = torch.tensor(c3["rho"], dtype=torch.float64, device=DEV, requires_grad=True)
+    comp(r).backward()
+    assert comp.last_stats.restarts >= 2           # a solve from zero, not a warm-start return
+    assert np.abs(r.grad.cpu().numpy() - c3["grad"]).max() < 1e-8 * np.abs(c3["grad"]).max()

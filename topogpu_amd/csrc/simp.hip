@@ -107,6 +107,117 @@ __global__ void k_sensitivity(Grid g, const double* __restrict__ U, const double
   dcf[e] = dh ? dcp * dh[e] : dcp;
 }
 
+// ---- compliance in double-double precision (finite-difference checks)
+// Error-free transformations: a + b = s + e exactly (TwoSum), a * b = p + e
+// exactly (fma TwoProduct).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+  p = a * b;
+  e = fma(a, b, -p);
+}
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ void dd_add(DD& x, double y) {
+  double s, e;
+  two_sum(x.hi, y, s, e);
+  x.hi = s;
+  x.lo += e;
+}
+__device__ __forceinline__ void dd_add(DD& x, const DD& y) {
+  double s, e;
+  two_sum(x.hi, y.hi, s, e);
+  x.hi = s;
+  x.lo += e + y.lo;
+}
+
+// Per block: F.u and u^T K u = sum_e E_e d_e^T KE d_e (d_e = u_e minus the
+// element's first-node displacement; KE annihilates translations) with every
+// product and sum carried in double-double. Partials: 4 doubles per block.
+__global__ void __launch_bounds__(256) k_compliance_dd(Grid g, const double* __restrict__ E,
+                                                      const double* __restrict__ U, const double* __restrict__ F,
+                                                      Workspace ws) {
+  DD fu{0.0, 0.0}, uku{0.0, 0.0};
+  const long long nd = 3 * g.nodes(), ne = g.elems();
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nd; i += stride) {
+    double p, e;
+    two_prod(F[i], U[i], p, e);
+    dd_add(fu, p);
+    fu.lo += e;
+  }
+  for (long long el = blockIdx.x * (long long)blockDim.x + threadIdx.x; el < ne; el += stride) {
+    const int k = (int)(el % g.nz), j = (int)((el / g.nz) % g.ny), i = (int)(el / ((long long)g.nz * g.ny));
+    // d = u_e - u_node0 exactly as d + dl (TwoSum): the quadratic form is
+    // then d^T KE d + 2 dl^T KE d to second order in the rounding of d
+    double d[24], dl[24];
+    const double* u0 = U + 3 * g.node(i, j, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const double* pc = U + 3 * g.node(i + corner_x(c), j + corner_y(c), k + corner_z(c));
+#pragma unroll
+      for (int q = 0; q < 3; ++q) two_sum(pc[q], -u0[q], d[3 * c + q], dl[3 * c + q]);
+    }
+    DD w{0.0, 0.0};
+    double corr = 0.0;
+    for (int a = 3; a < 24; ++a)        // first-node rows and columns of d are zero
+      for (int b = 3; b < 24; ++b) {
+        const double kab = c_ke64[a * 24 + b];
+        double p, e, p2, e2;
+        two_prod(d[a], d[b], p, e);
+        two_prod(p, kab, p2, e2);
+        dd_add(w, p2);
+        w.lo += e2 + e * kab;
+        corr = fma(2.0 * dl[a] * kab, d[b], corr);
+      }
+    w.lo += corr;
+    double p, e;
+    two_prod(w.hi, E[el], p, e);
+    dd_add(uku, p);
+    uku.lo += e + w.lo * E[el];
+  }
+  // block reduction of the two pairs in fixed order
+  __shared__ DD s_fu[256], s_uku[256];
+  s_fu[threadIdx.x] = fu;
+  s_uku[threadIdx.x] = uku;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      dd_add(s_fu[threadIdx.x], s_fu[threadIdx.x + off]);
+      dd_add(s_uku[threadIdx.x], s_uku[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* pt = ws.partials + 4 * (size_t)blockIdx.x;
+    pt[0] = s_fu[0].hi;
+    pt[1] = s_fu[0].lo;
+    pt[2] = s_uku[0].hi;
+    pt[3] = s_uku[0].lo;
+  }
+}
+
+__global__ void k_compliance_dd_finalize(int nb, double* out, Workspace ws) {
+  if (threadIdx.x != 0) return;
+  DD fu{0.0, 0.0}, uku{0.0, 0.0};
+  for (int b = 0; b < nb; ++b) {
+    const double* pt = ws.partials + 4 * (size_t)b;
+    dd_add(fu, DD{pt[0], pt[1]});
+    dd_add(uku, DD{pt[2], pt[3]});
+  }
+  double s, e;
+  two_sum(fu.hi, fu.lo, s, e);
+  out[0] = s;
+  out[1] = e;
+  two_sum(uku.hi, uku.lo, s, e);
+  out[2] = s;
+  out[3] = e;
+}
+
 __device__ __forceinline__ double oc_candidate(double q, double lo, double hi, double lm) {
   // pub_simp_solver.py:326-328: clip to [rho-move, rho+move], then [1e-3, 1]
   const double x = q * (1.0 / sqrt(lm));   // same arithmetic as k_oc_pass
@@ -472,6 +583,20 @@ extern "C" int tg_sensitivity64(const tg_grid* gp, const double* U, const double
   const Grid g = to_grid(gp);
   hipLaunchKernelGGL(k_sensitivity, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, U,
                      rp, dh, penal, erange, ce, dcf);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_compliance_dd(const tg_grid* gp, const double* E, const double* U, const double* F, double* out,
+                                void* s) {
+  TG_NEED_INIT();
+  if (int r = check_grid(gp)) return r;
+  TG_REQUIRE(E && U && F && out, "compliance_dd: NULL argument");
+  const Grid g = to_grid(gp);
+  constexpr int nb = 1024;
+  hipLaunchKernelGGL(k_compliance_dd, dim3(nb), dim3(256), 0, (hipStream_t)s, g, E, U, F, workspace());
+  TG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_compliance_dd_finalize, dim3(1), dim3(64), 0, (hipStream_t)s, nb, out, workspace());
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

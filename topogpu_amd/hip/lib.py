@@ -72,6 +72,7 @@ _SIGS = {
     "tg_fill32": [_i64, _vp, _f32, _vp],
     "tg_fill64": [_i64, _vp, _f64, _vp],
     "tg_copy32": [_i64, _vp, _vp, _vp],
+    "tg_compliance_dd": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp],
     "tg_stencil64": [C.POINTER(TgGrid), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_density_chain": [_i64, _vp, _f64, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp],
     "tg_sensitivity64": [C.POINTER(TgGrid), _vp, _vp, _vp, _f64, _f64, _vp, _vp, _vp],
