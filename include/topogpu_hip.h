@@ -301,6 +301,42 @@ int tg_oc_finalize(int64_t n, const double* rho, const double* q, const double* 
                    const double* hi, const double* state, double* rho_new, double* change,
                    void* stream);
 
+/* ---- batches of right-hand sides (several load cases at once) ----------
+ * The multi-load case runs its load cases as one block through every solver
+ * kernel: each call below is the single-RHS entry point of the same name
+ * without the "_b" suffix applied to nrhs (1..64) right-hand sides in one
+ * launch (grid slice per RHS). A batched vector on a grid is nrhs contiguous
+ * slices of 3 * nodes (Krylov helpers: of n); per-RHS scalars (dot outputs,
+ * rz, pq, zq, rr, sumsq) are arrays of nrhs doubles. Operators, masks,
+ * moduli, dinv and the coarsest inverse are shared by the batch. The
+ * reference has no multi-RHS path; these replace running SolverV4 once per
+ * load case. */
+int tg_apply32_b(const tg_op32* op, int32_t nrhs, const float* u, float* y, const tg_ep32* ep,
+                 void* stream);
+int tg_apply64_b(const tg_grid* g, const uint8_t* mask, const double* E, int32_t nrhs, const double* u,
+                 const double* rhs, double* y, float* y32, double* sumsq, void* stream);
+int tg_restrict32_b(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c, int32_t nrhs,
+                    const float* r_fine, float* r_coarse, void* stream);
+int tg_prolong32_b(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, int32_t nrhs,
+                   const float* e_coarse, float* x_fine, int accumulate, void* stream);
+int tg_coarsest_apply_b(const double* Ainv, int32_t n_free, const int32_t* free_list, int64_t n_full,
+                        int32_t nrhs, const float* r_coarse, float* e_coarse, void* stream);
+int tg_dot32_b(int64_t n, int32_t nrhs, const float* a, const float* b, const float* c, double* out,
+               double* out2, void* stream);
+int tg_dot64_b(int64_t n, int32_t nrhs, const double* a, const double* b, double* out, void* stream);
+int tg_pcg_xr_b(int64_t n, int32_t nrhs, float* x, float* r, const float* p, const float* q,
+                const double* rz, const double* pq, double* rr, void* stream);
+int tg_pcg_dir_b(int64_t n, int32_t nrhs, float* p, const float* z, const double* zq, const double* pq,
+                 void* stream);
+int tg_cheb_init32_b(int64_t n, int32_t nrhs, const float* rhs, const float* dinv, float a, float* d,
+                     float* x, float* r, void* stream);
+/* Weighted multi-load sensitivity: ce = sum_r w_r u_r,e^T KE u_r,e (weights
+ * NULL = all 1), dcf as tg_sensitivity64 with that ce (the weighted
+ * compliance's gradient). */
+int tg_sensitivity64_b(const tg_grid* g, int32_t nrhs, const double* U, const double* weights,
+                       const double* rho_phys, const double* dh, double penal, double e_range,
+                       double* ce, double* dcf, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

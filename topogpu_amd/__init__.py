@@ -9,7 +9,7 @@ from __future__ import annotations
 __version__ = "0.1.0"
 
 from .evidence import EvidenceBundle  # noqa: F401
-from .problem import TopologyProblem, cantilever_3d, unit_cantilever  # noqa: F401
+from .problem import TopologyProblem, cantilever_3d, multi_load_bracket, unit_cantilever  # noqa: F401
 from .problem_spec import (  # noqa: F401
     DistributedLoad,
     EdgeSupport,
@@ -20,7 +20,8 @@ from .problem_spec import (  # noqa: F401
 from .solver import OCOptimizer, OptimizationResult, SIMPModel, SIMPSolver, solve  # noqa: F401
 
 __all__ = [
-    "__version__", "EvidenceBundle", "TopologyProblem", "cantilever_3d", "unit_cantilever", "ProblemSpec",
+    "__version__", "EvidenceBundle", "TopologyProblem", "cantilever_3d", "multi_load_bracket", "unit_cantilever",
+    "ProblemSpec",
     "PointSupport", "EdgeSupport", "PointLoad", "DistributedLoad", "SIMPSolver", "SIMPModel", "OCOptimizer",
     "OptimizationResult", "solve",
 ]

@@ -107,9 +107,15 @@ __global__ void k_fill64(long long n, double* x, double v) {
     x[i] = v;
 }
 
+// Batched vector kernels: blockIdx.y selects the right-hand side; each RHS
+// is a contiguous slice of n entries (slice r starts at r * n). Per-RHS
+// scalars are arrays indexed by r.
 __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a, const float* b,
-                                                    const float* c, double* out, double* out2,
-                                                    Workspace ws) {
+                                                    const float* c, Workspace ws) {
+  const long long o = blockIdx.y * n;
+  a += o;
+  b += o;
+  if (c) c += o;
   double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -122,6 +128,9 @@ __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a
 
 __global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* a, const double* b,
                                                     Workspace ws) {
+  const long long o = blockIdx.y * n;
+  a += o;
+  b += o;
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -129,22 +138,32 @@ __global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* 
   block_partials<1>(v, ws);
 }
 
-// One block: out_k = sum_b partials[k * nb + b] in block order.
-__global__ void __launch_bounds__(256) k_finalize(int K, int nb, const double* partials, double* out0, double* out1) {
+// Block r: out_k[r] = sum_b partials[k * nseg * seg + r * seg + b] in block
+// order (the layout block_partials<K> writes for a grid of nseg slices of seg
+// blocks each).
+__global__ void __launch_bounds__(256) k_finalize(int K, int seg, const double* partials, double* out0, double* out1) {
+  const int r = blockIdx.x, nseg = gridDim.x;
   for (int k = 0; k < K; ++k) {
     double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[(size_t)k * nb + b];
+    const double* pk = partials + (size_t)k * nseg * seg + (size_t)r * seg;
+    for (int b = threadIdx.x; b < seg; b += blockDim.x) v[0] += pk[b];
     block_sum<1>(v);
     double* o = k == 0 ? out0 : out1;
-    if (threadIdx.x == 0 && o) o[0] = v[0];
+    if (threadIdx.x == 0 && o) o[r] = v[0];
   }
 }
 
 __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, float* r, const float* p,
                                                      const float* q, const double* rz,
                                                      const double* pq, Workspace ws) {
-  const double den = pq[0];
-  const float alpha = (float)(den != 0.0 ? rz[0] / den : 0.0);
+  const int k = blockIdx.y;
+  const long long o = k * n;
+  x += o;
+  r += o;
+  p += o;
+  q += o;
+  const double den = pq[k];
+  const float alpha = (float)(den != 0.0 ? rz[k] / den : 0.0);
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -157,14 +176,24 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, flo
 }
 
 __global__ void k_pcg_dir(long long n, float* p, const float* z, const double* zq, const double* pq) {
-  const double den = pq[0];
-  const float beta = (float)(den != 0.0 ? -zq[0] / den : 0.0);
+  const int k = blockIdx.y;
+  const long long o = k * n;
+  p += o;
+  z += o;
+  const double den = pq[k];
+  const float beta = (float)(den != 0.0 ? -zq[k] / den : 0.0);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     p[i] = fmaf(beta, p[i], z[i]);
 }
 
+// dinv is shared by all right-hand sides (one operator).
 __global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, float a, float* d, float* x, float* r) {
+  const long long o = blockIdx.y * n;
+  rhs += o;
+  d += o;
+  x += o;
+  if (r) r += o;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const float v = a * dinv[i] * rhs[i];
@@ -311,41 +340,56 @@ extern "C" int tg_fill64(int64_t n, double* x, double v, void* s) {
   return TG_OK;
 }
 
-extern "C" int tg_dot32(int64_t n, const float* a, const float* b, const float* c, double* out,
-                        double* out2, void* s) {
+extern "C" int tg_dot32_b(int64_t n, int32_t nrhs, const float* a, const float* b, const float* c, double* out,
+                          double* out2, void* s) {
   TG_NEED_INIT();
-  TG_REQUIRE(n > 0 && a && b && out && (c == nullptr) == (out2 == nullptr), "bad dot32 args");
+  TG_REQUIRE(n > 0 && nrhs >= 1 && a && b && out && (c == nullptr) == (out2 == nullptr), "bad dot32 args");
   const int nb = vec_blocks(n);
-  hipLaunchKernelGGL(k_dot32, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, c, out, out2,
+  hipLaunchKernelGGL(k_dot32, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, c,
                      workspace());
   TG_LAUNCH_CHECK();
-  return finalize_sums(c ? 2 : 1, nb, out, out2, TG_STREAM(s));
+  return finalize_batched(c ? 2 : 1, nb, nrhs, out, out2, TG_STREAM(s));
+}
+extern "C" int tg_dot32(int64_t n, const float* a, const float* b, const float* c, double* out,
+                        double* out2, void* s) {
+  return tg_dot32_b(n, 1, a, b, c, out, out2, s);
+}
+extern "C" int tg_dot64_b(int64_t n, int32_t nrhs, const double* a, const double* b, double* out, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && nrhs >= 1 && a && b && out, "bad dot64 args");
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_dot64, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, workspace());
+  TG_LAUNCH_CHECK();
+  return finalize_batched(1, nb, nrhs, out, nullptr, TG_STREAM(s));
 }
 extern "C" int tg_dot64(int64_t n, const double* a, const double* b, double* out, void* s) {
+  return tg_dot64_b(n, 1, a, b, out, s);
+}
+extern "C" int tg_pcg_xr_b(int64_t n, int32_t nrhs, float* x, float* r, const float* p, const float* q,
+                           const double* rz, const double* pq, double* rr, void* s) {
   TG_NEED_INIT();
-  TG_REQUIRE(n > 0 && a && b && out, "bad dot64 args");
+  TG_REQUIRE(n > 0 && nrhs >= 1 && x && r && p && q && rz && pq && rr, "bad pcg_xr args");
   const int nb = vec_blocks(n);
-  hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, workspace());
+  hipLaunchKernelGGL(k_pcg_xr, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r, p, q, rz, pq,
+                     workspace());
   TG_LAUNCH_CHECK();
-  return finalize_sums(1, nb, out, nullptr, TG_STREAM(s));
+  return finalize_batched(1, nb, nrhs, rr, nullptr, TG_STREAM(s));
 }
 extern "C" int tg_pcg_xr(int64_t n, float* x, float* r, const float* p, const float* q,
                          const double* rz, const double* pq, double* rr, void* s) {
-  TG_NEED_INIT();
-  TG_REQUIRE(n > 0 && x && r && p && q && rz && pq && rr, "bad pcg_xr args");
-  const int nb = vec_blocks(n);
-  hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r, p, q, rz, pq,
-                     workspace());
-  TG_LAUNCH_CHECK();
-  return finalize_sums(1, nb, rr, nullptr, TG_STREAM(s));
+  return tg_pcg_xr_b(n, 1, x, r, p, q, rz, pq, rr, s);
 }
-extern "C" int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const double* pq,
-                          void* s) {
-  TG_REQUIRE(n > 0 && p && z && zq && pq, "bad pcg_dir args");
-  hipLaunchKernelGGL(k_pcg_dir, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, p, z,
+extern "C" int tg_pcg_dir_b(int64_t n, int32_t nrhs, float* p, const float* z, const double* zq, const double* pq,
+                            void* s) {
+  TG_REQUIRE(n > 0 && nrhs >= 1 && p && z && zq && pq, "bad pcg_dir args");
+  hipLaunchKernelGGL(k_pcg_dir, dim3(vec_blocks(n), nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, p, z,
                      zq, pq);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+extern "C" int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const double* pq,
+                          void* s) {
+  return tg_pcg_dir_b(n, 1, p, z, zq, pq, s);
 }
 extern "C" int tg_axpy_f32_to_f64(int64_t n, double* x, const float* d, void* s) {
   TG_REQUIRE(n > 0 && x && d, "bad axpy args");
@@ -355,13 +399,17 @@ extern "C" int tg_axpy_f32_to_f64(int64_t n, double* x, const float* d, void* s)
   return TG_OK;
 }
 
-extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, float* d, float* x, float* r,
-                              void* s) {
-  TG_REQUIRE(n > 0 && rhs && dinv && d && x && r != rhs, "bad cheb_init args");
-  hipLaunchKernelGGL(k_cheb_init, dim3(vec_blocks(n)), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, rhs, dinv, a,
-                     d, x, r);
+extern "C" int tg_cheb_init32_b(int64_t n, int32_t nrhs, const float* rhs, const float* dinv, float a, float* d,
+                                float* x, float* r, void* s) {
+  TG_REQUIRE(n > 0 && nrhs >= 1 && rhs && dinv && d && x && r != rhs, "bad cheb_init args");
+  hipLaunchKernelGGL(k_cheb_init, dim3(vec_blocks(n), nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, rhs,
+                     dinv, a, d, x, r);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, float* d, float* x, float* r,
+                              void* s) {
+  return tg_cheb_init32_b(n, 1, rhs, dinv, a, d, x, r, s);
 }
 extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv, void* s) {
   TG_NEED_INIT();
@@ -374,10 +422,14 @@ extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float
 }
 
 namespace tg {
-int finalize_sums(int K, int nb, double* out0, double* out1, hipStream_t s) {
-  TG_REQUIRE(K >= 1 && K <= 2 && nb >= 1 && (size_t)K * nb <= (size_t)kMaxPartials, "bad reduction shape");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, K, nb, workspace().partials, out0, out1);
+int finalize_batched(int K, int seg, int nseg, double* out0, double* out1, hipStream_t s) {
+  TG_REQUIRE(K >= 1 && K <= 2 && seg >= 1 && nseg >= 1 && (size_t)K * seg * nseg <= (size_t)kMaxPartials,
+             "bad reduction shape");
+  hipLaunchKernelGGL(k_finalize, dim3(nseg), dim3(256), 0, s, K, seg, workspace().partials, out0, out1);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+int finalize_sums(int K, int nb, double* out0, double* out1, hipStream_t s) {
+  return finalize_batched(K, nb, 1, out0, out1, s);
 }
 }  // namespace tg

@@ -33,6 +33,17 @@ struct Ep32 {
 
 enum EpVec { EV_RHS = 0, EV_DINV, EV_D, EV_X, EV_R, EV_Y, EV_COUNT };
 
+// The epilogue of right-hand side r of a batch: every per-RHS vector offset
+// by off = r * (3 * nodes); dinv belongs to the operator and is shared.
+__host__ __device__ __forceinline__ Ep32 ep_slice(Ep32 e, long long off) {
+  if (e.rhs) e.rhs += off;
+  if (e.d) e.d += off;
+  if (e.x_out) e.x_out += off;
+  if (e.r) e.r += off;
+  if (e.y) e.y += off;
+  return e;
+}
+
 template <int MODE>
 struct EpMode {
   struct In {
@@ -147,6 +158,7 @@ struct Ep32M : Ep32 {
     return store(node, m, au, u, in);
   }
   __host__ __device__ static constexpr bool reduces() { return Mode::kReduces; }
+  __device__ __forceinline__ Ep32M slice(long long off) const { return Ep32M{ep_slice(*this, off)}; }
 };
 
 // Call f(Ep32M<mode>{e}) for the runtime mode of e; returns f's status, or
@@ -172,6 +184,14 @@ struct Ep64 {
   double* y;
   float* y32;
   bool norm;
+
+  __device__ __forceinline__ Ep64 slice(long long off) const {
+    Ep64 e = *this;
+    if (e.rhs) e.rhs += off;
+    if (e.y) e.y += off;
+    if (e.y32) e.y32 += off;
+    return e;
+  }
 
   struct In {
     double p[3];

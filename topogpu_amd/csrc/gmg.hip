@@ -43,8 +43,12 @@ __host__ __device__ __forceinline__ double axis_w(int i, int I, int nc) {
   return w;
 }
 
+// Transfers of a batch of right-hand sides: blockIdx.y selects the RHS, whose
+// vectors are contiguous slices of 3 * nodes of their grid.
 __global__ void k_prolong(Grid f, Grid c, const uint8_t* __restrict__ mask_f, const float* __restrict__ ec,
                           float* __restrict__ xf, int accumulate) {
+  ec += (long long)blockIdx.y * 3 * c.nodes();
+  xf += (long long)blockIdx.y * 3 * f.nodes();
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= f.nodes()) return;
   const int k = (int)(n % f.NZ()), j = (int)((n / f.NZ()) % f.NY()), i = (int)(n / ((long long)f.NZ() * f.NY()));
@@ -69,6 +73,8 @@ __global__ void k_prolong(Grid f, Grid c, const uint8_t* __restrict__ mask_f, co
 
 __global__ void k_restrict(Grid f, Grid c, const uint8_t* __restrict__ mask_c, const float* __restrict__ rf,
                            float* __restrict__ rc) {
+  rf += (long long)blockIdx.y * 3 * f.nodes();
+  rc += (long long)blockIdx.y * 3 * c.nodes();
   const long long N = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (N >= c.nodes()) return;
   const int K = (int)(N % c.NZ()), J = (int)((N / c.NZ()) % c.NY()), I = (int)(N / ((long long)c.NZ() * c.NY()));
@@ -201,9 +207,12 @@ __global__ void k_symmetrize_from_upper(double* A, int n) {
   if (r > c) A[t] = A[(long long)c * n + r];
 }
 
-// One wavefront per row: ec[free[i]] = sum_j Ainv[i][j] rc[free[j]].
+// One wavefront per row: ec[free[i]] = sum_j Ainv[i][j] rc[free[j]]; RHS
+// blockIdx.y of a batch uses the slices at blockIdx.y * n_full.
 __global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const int* __restrict__ fl,
-                                 const float* __restrict__ rc, float* __restrict__ ec) {
+                                 long long n_full, const float* __restrict__ rc, float* __restrict__ ec) {
+  rc += blockIdx.y * n_full;
+  ec += blockIdx.y * n_full;
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
@@ -417,26 +426,34 @@ static int check_pair(const tg_grid* fine, const tg_grid* coarse) {
   return TG_OK;
 }
 
-extern "C" int tg_restrict32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c, const float* rf,
-                             float* rc, void* s) {
+extern "C" int tg_restrict32_b(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c, int32_t nrhs,
+                               const float* rf, float* rc, void* s) {
   if (int r = check_pair(fine, coarse)) return r;
-  TG_REQUIRE(mask_c && rf && rc, "restrict: NULL argument");
+  TG_REQUIRE(mask_c && rf && rc && nrhs >= 1 && nrhs <= 64, "restrict: bad argument");
   const Grid f = to_grid(fine), c = to_grid(coarse);
-  hipLaunchKernelGGL(k_restrict, dim3((unsigned)((c.nodes() + 255) / 256)), dim3(256), 0, (hipStream_t)s, f, c,
-                     mask_c, rf, rc);
+  hipLaunchKernelGGL(k_restrict, dim3((unsigned)((c.nodes() + 255) / 256), nrhs), dim3(256), 0, (hipStream_t)s, f,
+                     c, mask_c, rf, rc);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
+extern "C" int tg_restrict32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_c, const float* rf,
+                             float* rc, void* s) {
+  return tg_restrict32_b(fine, coarse, mask_c, 1, rf, rc, s);
+}
 
-extern "C" int tg_prolong32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, const float* ec,
-                            float* xf, int accumulate, void* s) {
+extern "C" int tg_prolong32_b(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, int32_t nrhs,
+                              const float* ec, float* xf, int accumulate, void* s) {
   if (int r = check_pair(fine, coarse)) return r;
-  TG_REQUIRE(mask_f && ec && xf, "prolong: NULL argument");
+  TG_REQUIRE(mask_f && ec && xf && nrhs >= 1 && nrhs <= 64, "prolong: bad argument");
   const Grid f = to_grid(fine), c = to_grid(coarse);
-  hipLaunchKernelGGL(k_prolong, dim3((unsigned)((f.nodes() + 255) / 256)), dim3(256), 0, (hipStream_t)s, f, c,
-                     mask_f, ec, xf, accumulate);
+  hipLaunchKernelGGL(k_prolong, dim3((unsigned)((f.nodes() + 255) / 256), nrhs), dim3(256), 0, (hipStream_t)s, f,
+                     c, mask_f, ec, xf, accumulate);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+extern "C" int tg_prolong32(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, const float* ec,
+                            float* xf, int accumulate, void* s) {
+  return tg_prolong32_b(fine, coarse, mask_f, 1, ec, xf, accumulate, s);
 }
 
 extern "C" int tg_galerkin_elems(const tg_grid* fine, const tg_grid* coarse, const uint8_t* mask_f, int use_mask,
@@ -512,14 +529,19 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   return TG_OK;
 }
 
-extern "C" int tg_coarsest_apply(const double* Ainv, int32_t n, const int32_t* fl, int64_t n_full, const float* rc,
-                                 float* ec, void* s) {
-  TG_REQUIRE(Ainv && fl && rc && ec && n > 0 && n_full >= n, "coarsest_apply: bad argument");
-  if (int r = tg_fill32(n_full, ec, 0.0f, s)) return r;
-  hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)s, Ainv, n, fl, rc,
-                     ec);
+extern "C" int tg_coarsest_apply_b(const double* Ainv, int32_t n, const int32_t* fl, int64_t n_full, int32_t nrhs,
+                                   const float* rc, float* ec, void* s) {
+  TG_REQUIRE(Ainv && fl && rc && ec && n > 0 && n_full >= n && nrhs >= 1 && nrhs <= 64,
+             "coarsest_apply: bad argument");
+  if (int r = tg_fill32(n_full * nrhs, ec, 0.0f, s)) return r;
+  hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)((n + 3) / 4), nrhs), dim3(256), 0, (hipStream_t)s, Ainv, n,
+                     fl, (long long)n_full, rc, ec);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+extern "C" int tg_coarsest_apply(const double* Ainv, int32_t n, const int32_t* fl, int64_t n_full, const float* rc,
+                                 float* ec, void* s) {
+  return tg_coarsest_apply_b(Ainv, n, fl, n_full, 1, rc, ec, s);
 }
 
 extern "C" int tg_galerkin_from_fine(const tg_grid* fine, int down, const double* fine_E, double* work,

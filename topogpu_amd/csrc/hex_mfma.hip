@@ -132,10 +132,15 @@ struct Block {
   BufAcc acc;
 };
 
+// Right-hand side blockIdx.y of a batch: every per-RHS vector is offset by
+// blockIdx.y * 3 * nodes (dinv is the operator's and shared).
 template <class T>
 __device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const uint8_t* mask, const float* u,
-                                             const Ep32& ep) {
+                                             const Ep32& ep_all) {
   Block b;
+  const long long roff = (long long)blockIdx.y * 3 * g.nodes();
+  u += roff;
+  const Ep32 ep = ep_slice(ep_all, roff);
   b.NXn = g.nx + 1;
   b.NYn = g.ny + 1;
   b.NZn = g.nz + 1;
@@ -513,41 +518,43 @@ int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_
 }
 
 int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const float* u, const Ep32& ep,
-                    double* dot, bool split, hipStream_t s) {
+                    double* dot, bool split, int nrhs, hipStream_t s) {
   TileShape ts;
   long long nb;
   if (int rc = tile_launch_shape<Tile0>(g, ts, nb)) return rc;
+  TG_REQUIRE(nb * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
     if (split)
-      hipLaunchKernelGGL((k_hex_mfma<true, M>), dim3((unsigned)nb), dim3(Tile0::NTHR), 0, s, g, ts, mask, E, u,
-                         ep, workspace());
+      hipLaunchKernelGGL((k_hex_mfma<true, M>), dim3((unsigned)nb, nrhs), dim3(Tile0::NTHR), 0, s, g, ts, mask, E,
+                         u, ep, workspace());
     else
-      hipLaunchKernelGGL((k_hex_mfma<false, M>), dim3((unsigned)nb), dim3(Tile0::NTHR), 0, s, g, ts, mask, E, u,
-                         ep, workspace());
+      hipLaunchKernelGGL((k_hex_mfma<false, M>), dim3((unsigned)nb, nrhs), dim3(Tile0::NTHR), 0, s, g, ts, mask, E,
+                         u, ep, workspace());
     TG_LAUNCH_CHECK();
-    return EpMode<M>::kReduces ? finalize_sums(1, (int)nb, dot, nullptr, s) : TG_OK;
+    return EpMode<M>::kReduces ? finalize_batched(1, (int)nb, nrhs, dot, nullptr, s) : TG_OK;
   });
 }
 
 int launch_q8_mfma(const Grid& g, const uint8_t* mask, const float* Ef, const float* u, const Ep32& ep, double* dot,
-                   bool split, hipStream_t s) {
+                   bool split, int nrhs, hipStream_t s) {
   TileShape ts;
   long long nb;
   if (int rc = tile_launch_shape<Tile1>(g, ts, nb)) return rc;
+  TG_REQUIRE(nb * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
   // fine-grid slab offsets: NW element columns = 2 NW fine x-planes
   TG_REQUIRE((long long)(2 * Tile1::NW) * (2 * g.ny) * (2 * g.nz) * 4 < (long long)kMaxRange,
              "grid cross-section too large for 32-bit slab offsets");
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
     if (split)
-      hipLaunchKernelGGL((k_q8_mfma<true, M>), dim3((unsigned)nb), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef, u, ep,
-                         workspace());
+      hipLaunchKernelGGL((k_q8_mfma<true, M>), dim3((unsigned)nb, nrhs), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef, u,
+                         ep, workspace());
     else
-      hipLaunchKernelGGL((k_q8_mfma<false, M>), dim3((unsigned)nb), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef, u, ep,
-                         workspace());
+      hipLaunchKernelGGL((k_q8_mfma<false, M>), dim3((unsigned)nb, nrhs), dim3(Tile1::NTHR), 0, s, g, ts, mask, Ef,
+                         u, ep, workspace());
     TG_LAUNCH_CHECK();
-    return EpMode<M>::kReduces ? finalize_sums(1, (int)nb, dot, nullptr, s) : TG_OK;
+    return EpMode<M>::kReduces ? finalize_batched(1, (int)nb, nrhs, dot, nullptr, s) : TG_OK;
   });
 }
 

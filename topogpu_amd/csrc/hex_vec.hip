@@ -19,12 +19,12 @@
 namespace tg {
 
 int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const float* u, const Ep32& ep,
-                    double* dot, bool split, hipStream_t s);
+                    double* dot, bool split, int nrhs, hipStream_t s);
 int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s);
 int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
-                   double* dot, hipStream_t s);
+                   double* dot, int nrhs, hipStream_t s);
 int launch_q8_mfma(const Grid& g, const uint8_t* mask, const float* Ef, const float* u, const Ep32& ep, double* dot,
-                   bool split, hipStream_t s);
+                   bool split, int nrhs, hipStream_t s);
 
 namespace {
 
@@ -100,11 +100,15 @@ __device__ __forceinline__ void load_node_brick(const Grid& g, const uint8_t* __
 // Generic node-centric operator with element weights W.
 template <typename T, class W, class Ep>
 __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restrict__ mask, W w,
-                                               const T* __restrict__ u, Ep ep, double* dot,
+                                               const T* __restrict__ u, Ep ep_all, double* dot,
                                                Workspace ws) {
   constexpr int NS = W::NS;
   __shared__ T su[BX * BY * BZ * 3];
   __shared__ T sw[EX * EY * EZ * NS];
+  // right-hand side blockIdx.y of a batch
+  const long long roff = (long long)blockIdx.y * 3 * g.nodes();
+  u += roff;
+  const Ep ep = ep_all.slice(roff);
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
   load_node_brick<T>(g, mask, u, i0, j0, k0, su);
@@ -174,9 +178,12 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
 template <class Ep>
 __global__ void __launch_bounds__(NT) k_node_stored(Grid g, const uint8_t* __restrict__ mask,
                                                    const float* __restrict__ mats,
-                                                   const float* __restrict__ u, Ep ep, double* dot,
+                                                   const float* __restrict__ u, Ep ep_all, double* dot,
                                                    Workspace ws) {
   __shared__ float su[BX * BY * BZ * 3];
+  const long long roff = (long long)blockIdx.y * 3 * g.nodes();
+  u += roff;
+  const Ep ep = ep_all.slice(roff);
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
   load_node_brick<float>(g, mask, u, i0, j0, k0, su);
@@ -261,13 +268,13 @@ __global__ void k_diag(Grid g, const uint8_t* __restrict__ mask, int kind, const
 }
 
 template <typename T, class W, class Ep>
-int launch_node(const Grid& g, const uint8_t* mask, W w, const T* u, const Ep& ep, double* dot,
+int launch_node(const Grid& g, const uint8_t* mask, W w, const T* u, const Ep& ep, double* dot, int nrhs,
                 hipStream_t s) {
   const int nb = node_tiles(g);
-  TG_REQUIRE(nb <= kMaxPartials, "grid too large for the reduction workspace");
-  hipLaunchKernelGGL((k_node_op<T, W, Ep>), dim3(nb), dim3(NT), 0, s, g, mask, w, u, ep, dot, workspace());
+  TG_REQUIRE((long long)nb * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
+  hipLaunchKernelGGL((k_node_op<T, W, Ep>), dim3(nb, nrhs), dim3(NT), 0, s, g, mask, w, u, ep, dot, workspace());
   TG_LAUNCH_CHECK();
-  return ep.reduces() ? finalize_sums(1, nb, dot, nullptr, s) : TG_OK;
+  return ep.reduces() ? finalize_batched(1, nb, nrhs, dot, nullptr, s) : TG_OK;
 }
 
 }  // namespace
@@ -277,9 +284,11 @@ using namespace tg;
 
 #define TG_NEED_INIT() TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device")
 
-extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_ep32* ep, void* stream) {
+extern "C" int tg_apply32_b(const tg_op32* op, int32_t nrhs, const float* u, float* y, const tg_ep32* ep,
+                            void* stream) {
   TG_NEED_INIT();
   TG_REQUIRE(op && ep && u, "tg_apply32: NULL operator, epilogue or input");
+  TG_REQUIRE(nrhs >= 1 && nrhs <= 64, "tg_apply32: nrhs must be 1..64");
   if (int rc = check_grid(&op->grid)) return rc;
   TG_REQUIRE(op->mask, "tg_apply32: mask is NULL");
   const Grid g = to_grid(&op->grid);
@@ -309,17 +318,17 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
     case TG_OP_HEX_F32:
       TG_REQUIRE(op->weights, "weights NULL");
       return with_mode(e, [&](const auto& em) {
-        return launch_node<float>(g, op->mask, HexW<float>{op->weights}, u, em, ep->dot, s);
+        return launch_node<float>(g, op->mask, HexW<float>{op->weights}, u, em, ep->dot, nrhs, s);
       });
     case TG_OP_HEX_BF16:
     case TG_OP_HEX_BF16X3:
       TG_REQUIRE(op->weights, "weights NULL");
-      return launch_hex_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_HEX_BF16X3, s);
+      return launch_hex_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_HEX_BF16X3, nrhs, s);
     case TG_OP_Q8_F32: {
       TG_REQUIRE(op->weights, "weights NULL");
       const Grid fine{2 * g.nx, 2 * g.ny, 2 * g.nz};
       return with_mode(e, [&](const auto& em) {
-        return launch_node<float>(g, op->mask, Q8W{op->weights, fine}, u, em, ep->dot, s);
+        return launch_node<float>(g, op->mask, Q8W{op->weights, fine}, u, em, ep->dot, nrhs, s);
       });
     }
     case TG_OP_STORED_F32: {
@@ -327,32 +336,42 @@ extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_
       const int nb = node_tiles(g);
       return with_mode(e, [&](const auto& em) {
         using E = std::decay_t<decltype(em)>;
-        hipLaunchKernelGGL((k_node_stored<E>), dim3(nb), dim3(NT), 0, s, g, op->mask, op->elem_mats, u, em,
+        TG_REQUIRE((long long)nb * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
+        hipLaunchKernelGGL((k_node_stored<E>), dim3(nb, nrhs), dim3(NT), 0, s, g, op->mask, op->elem_mats, u, em,
                            ep->dot, workspace());
         TG_LAUNCH_CHECK();
-        return E::reduces() ? finalize_sums(1, nb, ep->dot, nullptr, s) : TG_OK;
+        return E::reduces() ? finalize_batched(1, nb, nrhs, ep->dot, nullptr, s) : TG_OK;
       });
     }
     case TG_OP_Q8_BF16:
     case TG_OP_Q8_BF16X3:
       TG_REQUIRE(op->weights, "weights NULL");
-      return launch_q8_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_Q8_BF16X3, s);
+      return launch_q8_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_Q8_BF16X3, nrhs, s);
     case TG_OP_STENCIL_F32:
       TG_REQUIRE(op->stencil, "stencil NULL");
-      return launch_stencil(g, op->mask, op->stencil, u, e, ep->dot, s);
+      return launch_stencil(g, op->mask, op->stencil, u, e, ep->dot, nrhs, s);
     default:
       return set_error(TG_ERR_ARG, "unknown operator kind %d", op->kind);
   }
 }
 
-extern "C" int tg_apply64(const tg_grid* gp, const uint8_t* mask, const double* E, const double* u,
-                          const double* rhs, double* y, float* y32, double* sumsq, void* stream) {
+extern "C" int tg_apply32(const tg_op32* op, const float* u, float* y, const tg_ep32* ep, void* stream) {
+  return tg_apply32_b(op, 1, u, y, ep, stream);
+}
+
+extern "C" int tg_apply64_b(const tg_grid* gp, const uint8_t* mask, const double* E, int32_t nrhs, const double* u,
+                            const double* rhs, double* y, float* y32, double* sumsq, void* stream) {
   TG_NEED_INIT();
   if (int rc = check_grid(gp)) return rc;
   TG_REQUIRE(mask && E && u && (y || y32), "tg_apply64: NULL argument");
+  TG_REQUIRE(nrhs >= 1 && nrhs <= 64, "tg_apply64: nrhs must be 1..64");
   const Grid g = to_grid(gp);
   Ep64 e{rhs, y, y32, sumsq != nullptr};
-  return launch_node<double>(g, mask, HexW<double>{E}, u, e, sumsq, (hipStream_t)stream);
+  return launch_node<double>(g, mask, HexW<double>{E}, u, e, sumsq, nrhs, (hipStream_t)stream);
+}
+extern "C" int tg_apply64(const tg_grid* gp, const uint8_t* mask, const double* E, const double* u,
+                          const double* rhs, double* y, float* y32, double* sumsq, void* stream) {
+  return tg_apply64_b(gp, mask, E, 1, u, rhs, y, y32, sumsq, stream);
 }
 
 extern "C" int tg_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, void* stream) {

@@ -66,5 +66,8 @@ __device__ __forceinline__ void block_partials(double (&v)[K], Workspace ws) {
 // Host side: launch the one-block finalize that sums K x nblocks partials
 // into out[0..K) (entries may be nullptr). Defined in core.hip.
 int finalize_sums(int K, int nblocks, double* out0, double* out1, hipStream_t s);
+// As finalize_sums for a grid of nseg slices of seg blocks each (one slice
+// per right-hand side): out_k[r] sums slice r. One block per slice.
+int finalize_batched(int K, int seg, int nseg, double* out0, double* out1, hipStream_t s);
 
 }  // namespace tg

@@ -79,27 +79,35 @@ __global__ void k_density_chain(long long n, const double* __restrict__ rf, doub
   if (E32) E32[e] = (float)E;
 }
 
-__global__ void k_sensitivity(Grid g, const double* __restrict__ U, const double* __restrict__ rp,
-                              const double* __restrict__ dh, double penal, double erange, double* ce_out,
-                              double* dcf) {
+// Weighted sum over a batch of load cases (U holds nrhs contiguous slices of
+// 3 * nodes; w = NULL weighs every case 1): ce = sum_r w_r u_r,e^T KE u_r,e.
+__global__ void k_sensitivity(Grid g, int nrhs, const double* __restrict__ U, const double* __restrict__ w,
+                              const double* __restrict__ rp, const double* __restrict__ dh, double penal,
+                              double erange, double* ce_out, double* dcf) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= g.elems()) return;
   const int k = (int)(e % g.nz), j = (int)((e / g.nz) % g.ny), i = (int)(e / ((long long)g.nz * g.ny));
-  double ue[24];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double* p = U + 3 * g.node(i + corner_x(c), j + corner_y(c), k + corner_z(c));
-    ue[3 * c] = p[0];
-    ue[3 * c + 1] = p[1];
-    ue[3 * c + 2] = p[2];
-  }
+  const long long stride = 3 * g.nodes();
   double ce = 0.0;
+  for (int r = 0; r < nrhs; ++r) {
+    const double* Ur = U + r * stride;
+    double ue[24];
 #pragma unroll
-  for (int a = 0; a < 24; ++a) {
-    double t = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const double* p = Ur + 3 * g.node(i + corner_x(c), j + corner_y(c), k + corner_z(c));
+      ue[3 * c] = p[0];
+      ue[3 * c + 1] = p[1];
+      ue[3 * c + 2] = p[2];
+    }
+    double cr = 0.0;
 #pragma unroll
-    for (int b = 0; b < 24; ++b) t = fma(c_ke64[a * 24 + b], ue[b], t);
-    ce = fma(ue[a], t, ce);
+    for (int a = 0; a < 24; ++a) {
+      double t = 0.0;
+#pragma unroll
+      for (int b = 0; b < 24; ++b) t = fma(c_ke64[a * 24 + b], ue[b], t);
+      cr = fma(ue[a], t, cr);
+    }
+    ce = fma(w ? w[r] : 1.0, cr, ce);
   }
   if (ce_out) ce_out[e] = ce;
   const double x = rp[e];
@@ -575,16 +583,21 @@ extern "C" int tg_density_chain(int64_t n, const double* rf, double beta, double
   return TG_OK;
 }
 
-extern "C" int tg_sensitivity64(const tg_grid* gp, const double* U, const double* rp, const double* dh,
-                                double penal, double erange, double* ce, double* dcf, void* s) {
+extern "C" int tg_sensitivity64_b(const tg_grid* gp, int32_t nrhs, const double* U, const double* weights,
+                                  const double* rp, const double* dh, double penal, double erange, double* ce,
+                                  double* dcf, void* s) {
   TG_NEED_INIT();
   if (int r = check_grid(gp)) return r;
-  TG_REQUIRE(U && rp && dcf, "sensitivity64: NULL argument");
+  TG_REQUIRE(U && rp && dcf && nrhs >= 1 && nrhs <= 64, "sensitivity64: bad argument");
   const Grid g = to_grid(gp);
-  hipLaunchKernelGGL(k_sensitivity, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, U,
-                     rp, dh, penal, erange, ce, dcf);
+  hipLaunchKernelGGL(k_sensitivity, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, nrhs,
+                     U, weights, rp, dh, penal, erange, ce, dcf);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+extern "C" int tg_sensitivity64(const tg_grid* gp, const double* U, const double* rp, const double* dh,
+                                double penal, double erange, double* ce, double* dcf, void* s) {
+  return tg_sensitivity64_b(gp, 1, U, nullptr, rp, dh, penal, erange, ce, dcf, s);
 }
 
 extern "C" int tg_compliance_dd(const tg_grid* gp, const double* E, const double* U, const double* F, double* out,

@@ -66,9 +66,12 @@ __global__ void __launch_bounds__(SNT) k_assemble_stencil(Grid g, const uint8_t*
 template <int MODE>
 __global__ void __launch_bounds__(SNT) k_stencil_apply(Grid g, const uint8_t* __restrict__ mask,
                                                       const float* __restrict__ coef, const float* __restrict__ u,
-                                                      Ep32 ep, Workspace ws) {
+                                                      Ep32 ep_all, Workspace ws) {
   using Mode = EpMode<MODE>;
   const long long nn = g.nodes();
+  // right-hand side blockIdx.z of a batch
+  u += (long long)blockIdx.z * 3 * nn;
+  const Ep32 ep = ep_slice(ep_all, (long long)blockIdx.z * 3 * nn);
   const long long n = blockIdx.x * (long long)SNT + threadIdx.x;
   const int c = blockIdx.y;
   const bool own = n < nn;
@@ -101,16 +104,16 @@ __global__ void __launch_bounds__(SNT) k_stencil_apply(Grid g, const uint8_t* __
 }  // namespace
 
 int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
-                   double* dot, hipStream_t s) {
+                   double* dot, int nrhs, hipStream_t s) {
   const long long nn = g.nodes();
   const long long nbx = (nn + SNT - 1) / SNT;
-  TG_REQUIRE(nbx * 3 <= kMaxPartials, "grid too large for the reduction workspace");
+  TG_REQUIRE(nbx * 3 * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
-    hipLaunchKernelGGL((k_stencil_apply<M>), dim3((unsigned)nbx, 3), dim3(SNT), 0, s, g, mask, coef, u, ep,
+    hipLaunchKernelGGL((k_stencil_apply<M>), dim3((unsigned)nbx, 3, nrhs), dim3(SNT), 0, s, g, mask, coef, u, ep,
                        workspace());
     TG_LAUNCH_CHECK();
-    return EpMode<M>::kReduces ? finalize_sums(1, (int)(nbx * 3), dot, nullptr, s) : TG_OK;
+    return EpMode<M>::kReduces ? finalize_batched(1, (int)(nbx * 3), nrhs, dot, nullptr, s) : TG_OK;
   });
 }
 

@@ -53,7 +53,7 @@ class HipCompliance:
         m.rho.copy_(rho.detach().reshape(-1))
         m.physical()
         if zero_start:
-            L.call("tg_fill64", m.fea.n, L.ptr(m.fea.u), 0.0, m.stream)
+            L.call("tg_fill64", m.fea.nrhs * m.fea.n, L.ptr(m.fea.u), 0.0, m.stream)
         c, stats = m.state()
         self.last_stats = stats
         if not stats.converged:
@@ -71,13 +71,18 @@ class HipCompliance:
 
     def variational_parts(self, rho: torch.Tensor) -> tuple[float, float, float, float]:
         """After a solve at rho: (F.u, u^T K u) as double-double pairs
-        (Fu_hi, Fu_lo, uKu_hi, uKu_lo)."""
+        (Fu_hi, Fu_lo, uKu_hi, uKu_lo); with several load cases, the weighted
+        sums over the cases (exact for unit weights)."""
         self._solve(rho, zero_start=False)
         m = self.model
         fea = m.fea
-        out = torch.zeros(4, dtype=torch.float64, device=m.dev)
-        L.call("tg_compliance_dd", fea.grid, L.ptr(fea.E64), L.ptr(fea.u), L.ptr(fea.F), L.ptr(out), m.stream)
-        return tuple(out.tolist())
+        out = torch.zeros(fea.nrhs, 4, dtype=torch.float64, device=m.dev)
+        for k in range(fea.nrhs):
+            L.call("tg_compliance_dd", fea.grid, L.ptr(fea.E64), L.ptr(fea._slice(fea.u, k)),
+                   L.ptr(fea._slice(fea.F, k)), L.ptr(out[k]), m.stream)
+        parts = out.cpu().numpy() * m.weights_host[:, None]
+        fu, uku = _dd_sum(parts[:, 0], parts[:, 1]), _dd_sum(parts[:, 2], parts[:, 3])
+        return fu + uku
 
     def variational(self, rho: torch.Tensor) -> float:
         """2 F.u - u^T K u after a solve at rho."""
@@ -93,6 +98,19 @@ class HipCompliance:
         fh_m, fl_m, kh_m, kl_m = parts_minus
         d = 2.0 * (fh_p - fh_m) - (kh_p - kh_m) + (2.0 * (fl_p - fl_m) - (kl_p - kl_m))
         return d / (2.0 * h)
+
+
+def _dd_sum(hi, lo) -> tuple[float, float]:
+    """Sum of double-double numbers (hi_k, lo_k) as one double-double pair
+    (Knuth's TwoSum on the high parts, low parts added to the error term)."""
+    s, e = 0.0, 0.0
+    for h, l in zip(hi.tolist(), lo.tolist()):
+        t = s + h
+        bv = t - s
+        e += (s - (t - bv)) + (h - bv) + l
+        s = t
+    t = s + e
+    return t, e - (t - s)
 
 
 class _ComplianceFn(torch.autograd.Function):

@@ -56,13 +56,14 @@ def hex_bytes(n_dof: int, n_elem: int, mode: int) -> float:
     return 4.0 * n_dof + 4.0 * n_elem + n_dof / 3.0 + _EP_VEC_TRAFFIC[mode] * n_dof
 
 
-def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: int) -> None:
-    """tg_apply32, timed with events when a timer is attached (level 0 only)."""
+def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: int, nrhs: int = 1) -> None:
+    """tg_apply32 over nrhs right-hand sides, accounted when a timer is
+    attached (level 0 only)."""
     if timer is None or lv != 0:
-        L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream)
+        L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream)
         return
-    timer.launch(KERNEL_NAMES[op.kind], hex_bytes(n_dof, n_elem, ep.mode),
-                 lambda: L.call("tg_apply32", op, u_ptr, y_ptr, ep, stream), ep.mode)
+    timer.launch(KERNEL_NAMES[op.kind], nrhs * hex_bytes(n_dof, n_elem, ep.mode),
+                 lambda: L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream), ep.mode)
 
 
 class GMG:
@@ -70,10 +71,14 @@ class GMG:
                  fine_accurate_op: int | None = None, l1_q8_op: int = L.TG_OP_Q8_F32,
                  l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
-                 power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8"):
+                 power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
+                 nrhs: int = 1):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
         self.dev = device
+        self.nrhs = int(nrhs)
+        if not 1 <= self.nrhs <= 64:
+            raise ValueError("nrhs must be 1..64")
         self.fine_degree = max(1, int(fine_degree))
         self.coarse_degree = max(1, int(coarse_degree))
         self.frac = float(cheb_lower_frac)
@@ -96,13 +101,14 @@ class GMG:
             Lv.n = 3 * Lv.nodes
             Lv.n_elem = int(np.prod(d))
             f32 = dict(dtype=torch.float32, device=device)
-            Lv.x = torch.zeros(Lv.n, **f32)       # iterate
-            Lv.r = torch.zeros(Lv.n, **f32)       # residual carried by the smoother
-            Lv.da = torch.zeros(Lv.n, **f32)      # smoother steps (ping-pong)
-            Lv.db = torch.zeros(Lv.n, **f32)
-            Lv.c = torch.zeros(Lv.n, **f32)       # prolonged coarse correction
-            Lv.rhs = torch.zeros(Lv.n, **f32)
-            Lv.dinv = torch.ones(Lv.n, **f32)
+            nb = self.nrhs * Lv.n                 # one slice of Lv.n per right-hand side
+            Lv.x = torch.zeros(nb, **f32)         # iterate
+            Lv.r = torch.zeros(nb, **f32)         # residual carried by the smoother
+            Lv.da = torch.zeros(nb, **f32)        # smoother steps (ping-pong)
+            Lv.db = torch.zeros(nb, **f32)
+            Lv.c = torch.zeros(nb, **f32)         # prolonged coarse correction
+            Lv.rhs = torch.zeros(nb, **f32)
+            Lv.dinv = torch.ones(Lv.n, **f32)     # shared by the batch
             Lv.v0 = torch.from_numpy(rng.standard_normal(Lv.n).astype(np.float32)).to(device)
             Lv.lam = float("nan")
             Lv.mats64 = None
@@ -160,9 +166,10 @@ class GMG:
             return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), None, None, L.ptr(Lv.stencil))
         return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), L.ptr(self.E32), None, None)
 
-    def _launch(self, lv, op, u, y, ep, stream):
+    def _launch(self, lv, op, u, y, ep, stream, nrhs=None):
         Lv = self.levels[lv]
-        launch_op(self.timer, lv, op, L.ptr(u), L.ptr(y), ep, stream, Lv.n, Lv.n_elem)
+        launch_op(self.timer, lv, op, L.ptr(u), L.ptr(y), ep, stream, Lv.n, Lv.n_elem,
+                  self.nrhs if nrhs is None else nrhs)
 
     # ---------------------------------------------------------------- setup
     def setup(self, E64: torch.Tensor, E32: torch.Tensor, emax: float, stream) -> None:
@@ -219,18 +226,18 @@ class GMG:
         no high-frequency content and 20 steps then underestimate lambda_max
         badly on large grids."""
         Lv = self.levels[lv]
-        v, kv = Lv.x, Lv.r
+        v, kv = Lv.x[:Lv.n], Lv.r[:Lv.n]
         v.copy_(Lv.v0)
-        lam = 1.0
         s = self.scal
         ep = L.ep(L.TG_EP_DOT, dot=s[0:1])
+        # the Rayleigh quotient is scale-free and v grows by about lambda_max
+        # per step, so the steps run back to back and only the last quotient
+        # is read
         for _ in range(iters):
-            self._launch(lv, self.op(lv, accurate=True), v, kv, ep, stream)
+            self._launch(lv, self.op(lv, accurate=True), v, kv, ep, stream, nrhs=1)
             L.call("tg_power_step32", Lv.n, L.ptr(v), L.ptr(kv), L.ptr(Lv.dinv), L.ptr(s[1:2]), stream)
-            vkv, vdv = s[0:2].tolist()
-            if vdv <= 1e-300:
-                break
-            lam = vkv / vdv
+        vkv, vdv = s[0:2].tolist()
+        lam = vkv / vdv if vdv > 1e-300 and np.isfinite(vkv / vdv) else 1.0
         return max(lam, 1e-6)
 
     # --------------------------------------------------------------- cycle
@@ -254,8 +261,8 @@ class GMG:
         """x from zero; leaves r = rhs - A x in Lv.r. Returns the last step buffer."""
         Lv = self.levels[lv]
         co = self._cheb_coeffs(lv, degree)
-        L.call("tg_cheb_init32", Lv.n, L.ptr(rhs), L.ptr(Lv.dinv), co[0][0], L.ptr(Lv.da), L.ptr(Lv.x), L.ptr(Lv.r),
-               stream)
+        L.call("tg_cheb_init32_b", Lv.n, self.nrhs, L.ptr(rhs), L.ptr(Lv.dinv), co[0][0], L.ptr(Lv.da), L.ptr(Lv.x),
+               L.ptr(Lv.r), stream)
         d, dn = Lv.da, Lv.db
         op = self.op(lv)
         for a1, a2 in co[1:]:
@@ -279,20 +286,22 @@ class GMG:
     def _vcycle(self, lv: int, rhs, stream):
         Lv = self.levels[lv]
         if lv == len(self.levels) - 1:
-            L.call("tg_coarsest_apply", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
-                   L.ptr(rhs), L.ptr(Lv.x), stream)
+            L.call("tg_coarsest_apply_b", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
+                   self.nrhs, L.ptr(rhs), L.ptr(Lv.x), stream)
             return Lv.x
         deg = self.fine_degree if lv == 0 else self.coarse_degree
         self._presmooth(lv, rhs, deg, stream)
         C = self.levels[lv + 1]
-        L.call("tg_restrict32", Lv.grid, C.grid, L.ptr(C.mask), L.ptr(Lv.r), L.ptr(C.rhs), stream)
+        L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
         ec = self._vcycle(lv + 1, C.rhs, stream)
-        L.call("tg_prolong32", Lv.grid, C.grid, L.ptr(Lv.mask), L.ptr(ec), L.ptr(Lv.c), 0, stream)
+        L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
         self._postsmooth(lv, deg, stream)
         return Lv.x
 
     def apply(self, r, stream):
-        """z = M r; returns a level-0 buffer valid until the next apply."""
+        """z = M r for every right-hand side of the batch (r: nrhs slices of
+        the level-0 length); returns a level-0 buffer valid until the next
+        apply."""
         if not self.is_setup:
             raise RuntimeError("GMG.setup() must be called before apply()")
         return self._vcycle(0, r, stream)

@@ -73,6 +73,17 @@ _SIGS = {
     "tg_fill64": [_i64, _vp, _f64, _vp],
     "tg_copy32": [_i64, _vp, _vp, _vp],
     "tg_compliance_dd": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp],
+    "tg_apply32_b": [C.POINTER(TgOp32), _i32, _vp, _vp, C.POINTER(TgEp32), _vp],
+    "tg_apply64_b": [C.POINTER(TgGrid), _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tg_restrict32_b": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, _i32, _vp, _vp, _vp],
+    "tg_prolong32_b": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, _i32, _vp, _vp, C.c_int, _vp],
+    "tg_coarsest_apply_b": [_vp, _i32, _vp, _i64, _i32, _vp, _vp, _vp],
+    "tg_dot32_b": [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tg_dot64_b": [_i64, _i32, _vp, _vp, _vp, _vp],
+    "tg_pcg_xr_b": [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tg_pcg_dir_b": [_i64, _i32, _vp, _vp, _vp, _vp, _vp],
+    "tg_cheb_init32_b": [_i64, _i32, _vp, _vp, _f32, _vp, _vp, _vp, _vp],
+    "tg_sensitivity64_b": [C.POINTER(TgGrid), _i32, _vp, _vp, _vp, _vp, _f64, _f64, _vp, _vp, _vp],
     "tg_stencil64": [C.POINTER(TgGrid), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_density_chain": [_i64, _vp, _f64, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp],
     "tg_sensitivity64": [C.POINTER(TgGrid), _vp, _vp, _vp, _f64, _f64, _vp, _vp, _vp],

@@ -34,13 +34,20 @@ class StepResult:
 class HipSIMP:
     def __init__(self, dims, free, F, *, volfrac: float, rmin: float = 1.5, penal: float = 3.0,
                  move: float = 0.2, use_heaviside: bool = True, beta: float = 1.0, eta: float = 0.5,
-                 rho0: np.ndarray | None = None, **solver_kw):
+                 rho0: np.ndarray | None = None, weights=None, **solver_kw):
         self.dims = tuple(int(d) for d in dims)
         self.n_elem = int(np.prod(self.dims))
         self.volfrac, self.rmin, self.penal, self.move = float(volfrac), float(rmin), float(penal), float(move)
         self.use_heaviside, self.beta, self.eta = bool(use_heaviside), float(beta), float(eta)
         self.fea = HipStateSolver(self.dims, free, F, **solver_kw)
         self.dev = self.fea.dev
+        # multi-load: the objective is sum_k w_k F_k.u_k (equal weights of 1
+        # by default, so it is the sum of the single-load compliances)
+        w = np.ones(self.fea.nrhs) if weights is None else np.asarray(weights, dtype=np.float64).ravel()
+        if w.size != self.fea.nrhs:
+            raise ValueError(f"weights must have one entry per load case ({self.fea.nrhs})")
+        self.weights_host = w.copy()
+        self.weights = torch.from_numpy(w.copy()).to(self.dev)
         self.grid = self.fea.grid
         offs, w = fem.filter_stencil(self.rmin)
         self.offs = np.ascontiguousarray(offs.ravel(), dtype=np.int32)
@@ -102,12 +109,13 @@ class HipSIMP:
         emax = float(self.E64.max().item())
         self.fea.set_modulus(self.E64, self.E32, emax)
         stats = self.fea.solve()
-        return self.fea.compliance(), stats
+        return float(self.weights_host @ self.fea.compliances()), stats
 
     def sensitivities(self) -> None:
-        """dc = H^T(dc_phys * dh), dv = H^T dh (pub_simp_solver.py:361-371)."""
-        L.call("tg_sensitivity64", self.grid, L.ptr(self.fea.u), L.ptr(self.rho_phys), L.ptr(self.dh), self.penal,
-               E0 - EMIN, None, L.ptr(self.dcf), self.stream)
+        """dc = H^T(dc_phys * dh), dv = H^T dh (pub_simp_solver.py:361-371);
+        with several load cases dc_phys is the weighted sum over the cases."""
+        L.call("tg_sensitivity64_b", self.grid, self.fea.nrhs, L.ptr(self.fea.u), L.ptr(self.weights),
+               L.ptr(self.rho_phys), L.ptr(self.dh), self.penal, E0 - EMIN, None, L.ptr(self.dcf), self.stream)
         self.filter(self.dcf, self.dc, transpose=True)
         self.filter(self.dh, self.dv, transpose=True)
 

@@ -53,12 +53,20 @@ PRECISIONS = {
 class SolveStats:
     iters: int            # preconditioner applications (the reference's count)
     restarts: int         # FP64 residual evaluations
-    converged: bool
-    rel_residual: float   # ||F - K u|| / ||F|| in FP64
+    converged: bool       # every right-hand side met its tolerance
+    rel_residual: float   # max over right-hand sides of ||F - K u|| / ||F|| in FP64
 
 
 class HipStateSolver:
-    """Device state solver for one structured grid and one load case."""
+    """Device state solver for one structured grid and one or more load cases.
+
+    ``F`` is one load vector (n,) or a batch (nrhs, n). A batch is solved as
+    nrhs independent PCG recurrences that share every launch: one operator
+    product, one V-cycle and one set of vector kernels per iteration serve all
+    load cases (the operator's moduli and the hierarchy are shared). Each load
+    case keeps its own Krylov scalars and tolerance; the block iterates until
+    every case has converged.
+    """
 
     def __init__(self, dims, free: np.ndarray, F: np.ndarray, *, precision: str = "bf16", n_levels="auto",
                  fine_degree: int = 2, coarse_degree: int = 2, cheb_lower_frac: float = 1.0 / 30.0,
@@ -82,10 +90,14 @@ class HipStateSolver:
         self.mask = torch.from_numpy(mask).to(self.dev)
         if n_levels == "auto":
             n_levels = fem.auto_levels(self.dims, mask)
-        F = np.asarray(F, dtype=np.float64).copy()
-        F[np.setdiff1d(np.arange(self.n), free)] = 0.0
-        self.F = torch.from_numpy(F).to(self.dev)
-        self.bnorm = float(np.linalg.norm(F))
+        F = np.array(F, dtype=np.float64, ndmin=2)
+        if F.ndim != 2 or F.shape[1] != self.n or not 1 <= F.shape[0] <= 64:
+            raise ValueError(f"F must have shape ({self.n},) or (nrhs, {self.n}) with 1 <= nrhs <= 64")
+        self.nrhs = int(F.shape[0])
+        F[:, np.setdiff1d(np.arange(self.n), free)] = 0.0
+        self.F = torch.from_numpy(F.ravel()).to(self.dev)
+        self.bnorms = np.linalg.norm(F, axis=1)
+        self.bnorm = float(self.bnorms.max())
         self.precision = precision
         self.tol, self.max_krylov = float(tol), int(max_krylov)
         self.inner_rtol, self.max_restarts = float(inner_rtol), int(max_restarts)
@@ -94,16 +106,18 @@ class HipStateSolver:
         self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[0],
                        l1_q8_op=ops[3], l1_q8_accurate_op=ops[2], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
-                       level1=level1, device=self.dev)
+                       level1=level1, nrhs=self.nrhs, device=self.dev)
+        nb = self.nrhs * self.n
         f32 = dict(dtype=torch.float32, device=self.dev)
-        self.u = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
-        self.r64 = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
-        self.r = torch.zeros(self.n, **f32)
-        self.d = torch.zeros(self.n, **f32)
-        self.p = torch.zeros(self.n, **f32)
-        self.q = torch.zeros(self.n, **f32)
-        # device scalars: 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 emax/compliance
-        self.s = torch.zeros(8, dtype=torch.float64, device=self.dev)
+        self.u = torch.zeros(nb, dtype=torch.float64, device=self.dev)
+        self.r64 = torch.zeros(nb, dtype=torch.float64, device=self.dev)
+        self.r = torch.zeros(nb, **f32)
+        self.d = torch.zeros(nb, **f32)
+        self.p = torch.zeros(nb, **f32)
+        self.q = torch.zeros(nb, **f32)
+        # device scalars, one row of nrhs per quantity:
+        # 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 compliance
+        self.s = torch.zeros(8, self.nrhs, dtype=torch.float64, device=self.dev)
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
@@ -111,6 +125,9 @@ class HipStateSolver:
     @property
     def stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _slice(self, t: torch.Tensor, k: int) -> torch.Tensor:
+        return t[k * self.n:(k + 1) * self.n]
 
     # ------------------------------------------------------------------
     def fine_op(self) -> L.TgOp32:
@@ -122,58 +139,65 @@ class HipStateSolver:
         self.E64, self.E32 = E64, E32
         self.gmg.setup(E64, E32, emax, self.stream)
 
-    def residual64(self) -> float:
-        """r64 = F - K u (FP64), r = float(r64); returns ||r64||."""
-        L.call("tg_apply64", self.grid, L.ptr(self.mask), L.ptr(self.E64), L.ptr(self.u), L.ptr(self.F),
-               L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4:5]), self.stream)
-        return math.sqrt(max(float(self.s[4].item()), 0.0))
+    def residual64(self) -> np.ndarray:
+        """r64 = F - K u (FP64), r = float(r64) for every load case; returns
+        the nrhs norms ||r64||."""
+        L.call("tg_apply64_b", self.grid, L.ptr(self.mask), L.ptr(self.E64), self.nrhs, L.ptr(self.u),
+               L.ptr(self.F), L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4]), self.stream)
+        return np.sqrt(np.maximum(self.s[4].cpu().numpy(), 0.0))
 
     def solve(self) -> SolveStats:
-        st = self.stream
+        st, n, nr = self.stream, self.n, self.nrhs
         if not self.warm_start:
-            L.call("tg_fill64", self.n, L.ptr(self.u), 0.0, st)
-        target = self.tol * self.bnorm
+            L.call("tg_fill64", nr * n, L.ptr(self.u), 0.0, st)
+        target = self.tol * self.bnorms
         its = restarts = 0
         rnorm = self.residual64()
         restarts += 1
-        if not np.isfinite(rnorm):
+        if not np.all(np.isfinite(rnorm)):
             raise FloatingPointError("non-finite residual at the start of the state solve")
-        if rnorm > 10.0 * self.bnorm and self.warm_start:
+        far = rnorm > 10.0 * self.bnorms
+        if self.warm_start and far.any():
             # a warm start that is far off is worse than zero (solver_v4.py:201-209)
-            L.call("tg_fill64", self.n, L.ptr(self.u), 0.0, st)
+            for k in np.flatnonzero(far):
+                L.call("tg_fill64", n, L.ptr(self._slice(self.u, k)), 0.0, st)
             rnorm = self.residual64()
             restarts += 1
         op = self.fine_op()
         s = self.s
-        while rnorm > target and its < self.max_krylov and restarts <= self.max_restarts:
-            inner_target = max(self.inner_rtol * rnorm, 0.5 * target)
-            L.call("tg_fill32", self.n, L.ptr(self.d), 0.0, st)
+        while np.any(rnorm > target) and its < self.max_krylov and restarts <= self.max_restarts:
+            inner_target = np.maximum(self.inner_rtol * rnorm, 0.5 * target)
+            L.call("tg_fill32", nr * n, L.ptr(self.d), 0.0, st)
             z = self.gmg.apply(self.r, st)
             its += 1
-            L.call("tg_dot32", self.n, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0:1]), None, st)
-            L.call("tg_copy32", self.n, L.ptr(self.p), L.ptr(z), st)
+            L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0]), None, st)
+            L.call("tg_copy32", nr * n, L.ptr(self.p), L.ptr(z), st)
             while True:
-                ep = L.ep(L.TG_EP_DOT, dot=s[1:2])
-                launch_op(self.gmg.timer, 0, op, L.ptr(self.p), L.ptr(self.q), ep, st, self.n, self.n_elem)
-                L.call("tg_pcg_xr", self.n, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
-                       L.ptr(s[0:1]), L.ptr(s[1:2]), L.ptr(s[2:3]), st)
-                rr = float(s[2].item())
-                if not np.isfinite(rr):
+                ep = L.ep(L.TG_EP_DOT, dot=s[1])
+                launch_op(self.gmg.timer, 0, op, L.ptr(self.p), L.ptr(self.q), ep, st, n, self.n_elem, nr)
+                L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
+                       L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
+                rr = s[2].cpu().numpy()
+                if not np.all(np.isfinite(rr)):
                     raise FloatingPointError("non-finite inner residual")
-                if math.sqrt(max(rr, 0.0)) <= inner_target or its >= self.max_krylov:
+                if np.all(np.sqrt(np.maximum(rr, 0.0)) <= inner_target) or its >= self.max_krylov:
                     break
                 z = self.gmg.apply(self.r, st)
                 its += 1
-                L.call("tg_dot32", self.n, L.ptr(z), L.ptr(self.r), L.ptr(self.q), L.ptr(s[0:1]), L.ptr(s[3:4]), st)
-                L.call("tg_pcg_dir", self.n, L.ptr(self.p), L.ptr(z), L.ptr(s[3:4]), L.ptr(s[1:2]), st)
-            L.call("tg_axpy_f32_to_f64", self.n, L.ptr(self.u), L.ptr(self.d), st)
+                L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), L.ptr(self.q), L.ptr(s[0]), L.ptr(s[3]), st)
+                L.call("tg_pcg_dir_b", n, nr, L.ptr(self.p), L.ptr(z), L.ptr(s[3]), L.ptr(s[1]), st)
+            L.call("tg_axpy_f32_to_f64", nr * n, L.ptr(self.u), L.ptr(self.d), st)
             rnorm = self.residual64()
             restarts += 1
-        self.last = SolveStats(its, restarts, bool(rnorm <= target), rnorm / max(self.bnorm, 1e-300))
+        rel = rnorm / np.maximum(self.bnorms, 1e-300)
+        self.last = SolveStats(its, restarts, bool(np.all(rnorm <= target)), float(rel.max()))
         return self.last
 
-    def compliance(self) -> float:
-        """F . u in FP64 (solver_v4.py:244)."""
-        L.call("tg_dot64", self.n, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5:6]), self.stream)
-        return float(self.s[5].item())
+    def compliances(self) -> np.ndarray:
+        """F_k . u_k in FP64 for every load case (solver_v4.py:244)."""
+        L.call("tg_dot64_b", self.n, self.nrhs, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5]), self.stream)
+        return self.s[5].cpu().numpy()
 
+    def compliance(self) -> float:
+        """F . u in FP64 (solver_v4.py:244); the sum over load cases of a batch."""
+        return float(self.compliances().sum())

@@ -87,3 +87,36 @@ def unit_cantilever(nel, volfrac: float = 0.30, rmin: float = 1.5, load: str = "
         raise ValueError("load must be 'point' or 'line'")
     return TopologyProblem(f"cantilever_{nelx}x{nely}x{nelz}_{load}", spec, filter_radius=rmin,
                            force_override=force, role="benchmark", metadata={"load": load})
+
+
+def multi_load_bracket(nel=(192, 64, 64), n_loads: int = 8, seed: int = 3, volfrac: float = 0.30,
+                       rmin: float = 1.5) -> TopologyProblem:
+    """Multi-load bracket: y=0 face clamped, ``n_loads`` load cases, each a
+    unit point load at a node of the y=nely face.
+
+    From ``np.random.default_rng(seed)``: the x indices of the nodes
+    (0..nelx), then their z indices (0..nelz), then one Gaussian triple per
+    case, normalised to unit length (a direction uniform on the sphere).
+    ``force_override`` holds the cases as rows, shape (n_loads, ndof); the
+    objective is the sum of their compliances. The reference solves one load
+    case at a time; this case runs them as one block of right-hand sides.
+    """
+    nelx, nely, nelz = nel
+    rng = np.random.default_rng(seed)
+    ix = rng.integers(0, nelx + 1, n_loads)
+    iz = rng.integers(0, nelz + 1, n_loads)
+    dirs = rng.standard_normal((n_loads, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    force = np.zeros((n_loads, ndof))
+    nodes = (ix * (nely + 1) + nely) * (nelz + 1) + iz
+    for k in range(n_loads):
+        force[k, 3 * nodes[k]: 3 * nodes[k] + 3] = dirs[k]
+    # the spec's own load (first case) keeps validate() and generate_bc() whole
+    spec = ProblemSpec(Lx=float(nelx), Ly=float(nely), Lz=float(nelz), nelx=nelx, nely=nely, nelz=nelz,
+                       volfrac=volfrac, supports=[EdgeSupport("bottom", "fixed")],
+                       loads=[PointLoad(x=float(ix[0]), y=float(nely), z=float(iz[0]), fx=dirs[0, 0],
+                                        fy=dirs[0, 1], fz=dirs[0, 2])], rmin=rmin)
+    return TopologyProblem(f"bracket_{nelx}x{nely}x{nelz}_{n_loads}loads", spec, filter_radius=rmin,
+                           force_override=force, role="benchmark",
+                           metadata={"n_loads": n_loads, "seed": seed, "nodes": nodes.tolist()})
