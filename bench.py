@@ -39,6 +39,11 @@ METRIC = "SIMP iter/s @1M hex8 (BF16-MFMA GMG vs FP32 plain, rtol 1e-5); matvec 
 C2 = dict(dims=(160, 80, 80), volfrac=0.30, rmin=1.5, move=0.2, n_levels="auto", fine_degree=3, coarse_degree=2,
           tol=1e-5, max_krylov=800)
 C1 = (128, 64, 64)
+# Multi-load bracket: 192x64x64, y=0 face clamped, 8 unit point loads on the
+# y=64 face (nodes and directions from default_rng(3)), equal weights, the 8
+# load cases solved as one block of right-hand sides; 30 OC iterations.
+C4 = dict(dims=(192, 64, 64), n_loads=8, seed=3, volfrac=0.30, rmin=1.5, move=0.2, n_levels="auto", fine_degree=3,
+          coarse_degree=2, tol=1e-5, max_krylov=800)
 HBM_PEAK_GBPS = 8000.0
 EP_NAMES = {0: "STORE", 1: "DOT", 2: "RESID", 3: "RESID_NORM", 4: "CHEB", 5: "CHEB_R", 6: "RESID_UPDATE"}
 
@@ -47,23 +52,60 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated: bool = False):
+# HBM bytes per launch of a level-0 (kernel, epilogue) pair at 160x80x80,
+# measured with hardware counters (FETCH_SIZE and WRITE_SIZE runs of
+# ``bench.py --kernel-probe``, scaled by the counts of a 256 MiB copy); the
+# summary file holds the commands and the figures.
+TRAFFIC_PROFILES = {"hex_bf16_mfma/CHEB_R": "profiles/hex_mfma_cheb_r_160x80x80.md"}
+
+
+def profiled_traffic(kernel: str):
+    path = TRAFFIC_PROFILES.get(kernel)
+    if path is None:
+        return None, None
+    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    try:
+        with open(full) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1]), path
+    except OSError:
+        pass
+    return None, None
+
+
+def make_engine(case: str, precision: str, n_levels=None, loads=None):
+    """HipSIMP for the 1M-element cantilever ("c2") or the multi-load
+    bracket ("c4"; ``loads`` selects a subset of its load cases)."""
+    import topogpu_amd as tg
+    from topogpu_amd.bc import generate_bc
+    from topogpu_amd.hip.simp import HipSIMP
+
+    if case == "c2":
+        cfg = C2
+        prob = tg.unit_cantilever(cfg["dims"], volfrac=cfg["volfrac"], rmin=cfg["rmin"], load="line")
+        force = prob.force_override
+    else:
+        cfg = C4
+        prob = tg.multi_load_bracket(cfg["dims"], n_loads=cfg["n_loads"], seed=cfg["seed"], volfrac=cfg["volfrac"],
+                                     rmin=cfg["rmin"])
+        force = prob.force_override if loads is None else prob.force_override[list(loads)]
+    bc = generate_bc(prob.spec)
+    return HipSIMP(cfg["dims"], bc.free_dofs, force, volfrac=cfg["volfrac"], rmin=cfg["rmin"], move=cfg["move"],
+                   precision=precision, n_levels=n_levels or cfg["n_levels"], fine_degree=cfg["fine_degree"],
+                   coarse_degree=cfg["coarse_degree"], tol=cfg["tol"], max_krylov=cfg["max_krylov"])
+
+
+def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated: bool = False, case: str = "c2",
+               loads=None):
     """Run warmup + steps SIMP iterations from rho = volfrac; time the steps.
     Level-0 operator launches are counted per (kernel, epilogue) in the timed
     region (no events: nothing but a counter is added to the loop)."""
     import torch
 
-    import topogpu_amd as tg
-    from topogpu_amd.bc import generate_bc
-    from topogpu_amd.hip.simp import HipSIMP
     from topogpu_amd.hip.timing import KernelTimer
 
-    prob = tg.unit_cantilever(C2["dims"], volfrac=C2["volfrac"], rmin=C2["rmin"], load="line")
-    bc = generate_bc(prob.spec)
-    eng = HipSIMP(C2["dims"], bc.free_dofs, prob.force_override, volfrac=C2["volfrac"], rmin=C2["rmin"],
-                  move=C2["move"], precision=precision, n_levels=n_levels or C2["n_levels"],
-                  fine_degree=C2["fine_degree"],
-                  coarse_degree=C2["coarse_degree"], tol=C2["tol"], max_krylov=C2["max_krylov"])
+    eng = make_engine(case, precision, n_levels, loads)
     for i in range(warmup):
         r = eng.step()
         _log(f"{precision} warmup {i + 1}/{warmup}: C={r.compliance:.6g} its={r.iters} conv={r.converged}")
@@ -90,7 +132,7 @@ def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated
         "ir_restarts_mean": float(np.mean([r.restarts for r in rows])),
         "max_rel_residual": float(max(r.rel_residual for r in rows)),
         "all_converged": bool(all(r.converged for r in rows)),
-        "compliance_last": float(rows[-1].compliance),
+        "compliance_last": float(rows[-1].compliance), "n_loads": eng.fea.nrhs,
     }
     launches = {f"{name}/{EP_NAMES[mode]}": n / steps for (name, mode), n in counter.counts.items()}
     kernels = {}
@@ -154,17 +196,10 @@ def kernel_probe(precision: str, reps: int = 20) -> dict:
     operands: the pair a counter run scales its byte counts with."""
     import torch
 
-    import topogpu_amd as tg
-    from topogpu_amd.bc import generate_bc
     from topogpu_amd.hip import lib as L
     from topogpu_amd.hip.gmg import hex_bytes
-    from topogpu_amd.hip.simp import HipSIMP
 
-    prob = tg.unit_cantilever(C2["dims"], volfrac=C2["volfrac"], rmin=C2["rmin"], load="line")
-    bc = generate_bc(prob.spec)
-    eng = HipSIMP(C2["dims"], bc.free_dofs, prob.force_override, volfrac=C2["volfrac"], rmin=C2["rmin"],
-                  move=C2["move"], precision=precision, n_levels=C2["n_levels"], fine_degree=C2["fine_degree"],
-                  coarse_degree=C2["coarse_degree"], tol=C2["tol"], max_krylov=C2["max_krylov"])
+    eng = make_engine("c2", precision)
     for _ in range(2):
         eng.step()
     copy_bytes = 256 << 20
@@ -309,6 +344,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-matvec", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the fixed 4-level hierarchy run")
+    ap.add_argument("--no-multi-load", action="store_true", help="skip the 8-load bracket block run")
     ap.add_argument("--levels", default=None, help="override the hierarchy depth of the headline run (int or auto)")
     ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -346,13 +382,28 @@ def main() -> int:
         st, _, _, _ = simp_bench(args.precision, args.steps, args.warmup, n_levels=4)
         variants[f"{args.precision}_4_levels"] = st
         _log(f"{args.precision} 4 levels: {st['iter_per_s']:.3f} iter/s")
+    multi = None
+    if not args.no_multi_load:
+        # 30 OC iterations of the 8-load bracket (2 warm-up + 28 timed), the
+        # block of 8 right-hand sides against its first load case alone
+        blk, _, _, sz4 = simp_bench(args.precision, 28, 2, case="c4")
+        one, _, _, _ = simp_bench(args.precision, 28, 2, case="c4", loads=[0])
+        multi = {"workload": "multi-load bracket 192x64x64 hex8 (786,432 elem), y=0 clamped, 8 unit point loads "
+                             "on the y=64 face (default_rng(3)), equal weights, 8-RHS block PCG-GMG, tol 1e-5, "
+                             "OC iterations 3-30 timed",
+                 "block_8_loads": blk, "single_load_0": one, "ndof": sz4["ndof"],
+                 "load_case_solves_per_s": 8 * blk["iter_per_s"],
+                 "block_cost_in_single_loads": one["iter_per_s"] / blk["iter_per_s"]}
+        _log(f"multi-load: block {blk['iter_per_s']:.3f} iter/s, single load {one['iter_per_s']:.3f} iter/s")
     mv = None if args.no_matvec else matvec_bench()
     # dominant level-0 (kernel, epilogue): most device time per SIMP step
     for k, v in kernels.items():
         v["launches_per_step"] = launches.get(k, 0.0)
     dom_name, dom = max(kernels.items(), key=lambda kv: kv[1]["avg_us"] * kv[1]["launches_per_step"])
+    traffic, traffic_src = profiled_traffic(dom_name) if C2["dims"] == (160, 80, 80) else (None, None)
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": dom["gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                "unit": "GB/s", "frac": dom["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "avg_us": dom["avg_us"], "launches_per_step": dom["launches_per_step"],
                 "algorithmic_bytes_per_launch": dom["bytes_per_launch"]}
     fine_ops = sum(launches.values())
@@ -371,7 +422,7 @@ def main() -> int:
         "solve": {k: bf[k] for k in ("pcg_iters_mean", "pcg_iters_max", "ir_restarts_mean", "max_rel_residual",
                                      "all_converged", "compliance_last")},
         "fp32_plain": fp, "speedup_vs_fp32": (bf["iter_per_s"] / fp["iter_per_s"]) if fp else None,
-        "variants": variants, "level0_launches_per_step": launches, "matvec": mv, "kernels": kernels,
+        "variants": variants, "multi_load": multi, "level0_launches_per_step": launches, "matvec": mv, "kernels": kernels,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
