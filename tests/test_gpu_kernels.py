@@ -117,6 +117,57 @@ def test_rigid_translation_is_exact_zero(prec):
         assert np.abs(got).max() == 0.0
 
 
+@pytest.mark.parametrize("dims", [(128, 64, 64), (141, 33, 95)])
+def test_hex_mfma_grids_larger_than_one_wave_of_workgroups(dims):
+    """Grids with more tiles than workgroups resident at once (the level-0
+    MFMA operator then runs as a persistent, pipelined kernel): every tile
+    is computed once, against the FP64 device operator (itself checked
+    against the oracle above), for STORE, DOT and CHEB_R, one and two
+    right-hand sides."""
+    rng = np.random.default_rng(31)
+    nn = int(np.prod([d + 1 for d in dims]))
+    n = 3 * nn
+    mask = np.full(nn, 7, np.uint8)
+    mask.reshape(dims[0] + 1, -1)[0] = 0
+    mask[rng.choice(nn, 100, replace=False)] = 2
+    E = rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3
+    dm, dE64, dE32 = t(mask), t(E), t(E.astype(np.float32))
+    u = rng.standard_normal(2 * n).astype(np.float32)
+    du = t(u)
+    y64 = torch.zeros(2 * n, dtype=torch.float64, device=DEV)
+    for k in range(2):
+        L.call("tg_apply64", L.grid(dims), L.ptr(dm), L.ptr(dE64), L.ptr(t(u[k * n:(k + 1) * n].astype(np.float64))),
+               None, L.ptr(y64[k * n:(k + 1) * n]), None, None, stream())
+    ref = h(y64)
+    assert np.abs(ref).max() > 0
+    for kind, tol in ((L.TG_OP_HEX_BF16X3, 1e-5), (L.TG_OP_HEX_BF16, 1e-2)):
+        op = L.TgOp32(kind, L.grid(dims), L.ptr(dm), L.ptr(dE32), None)
+        for nr in (1, 2):
+            y = torch.full((nr * n,), np.nan, dtype=torch.float32, device=DEV)
+            dot = torch.zeros(nr, dtype=torch.float64, device=DEV)
+            L.call("tg_apply32_b", op, nr, L.ptr(du), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), stream())
+            got, gd = h(y), h(dot)
+            for k in range(nr):
+                sl = slice(k * n, (k + 1) * n)
+                assert rel_l2(got[sl], ref[sl]) < tol, (kind, nr, k)
+                assert gd[k] == pytest.approx(float(u[sl].astype(float) @ ref[sl]), rel=10 * tol)
+        # CHEB_R: r -= A u ; d = a1 dinv r + a2 u ; x += a3 u + d
+        dinv = rng.uniform(0.5, 2.0, n).astype(np.float32)
+        r0 = rng.standard_normal(n).astype(np.float32)
+        x0 = rng.standard_normal(n).astype(np.float32)
+        dr, dx, dd = t(r0), t(x0), torch.full((n,), np.nan, dtype=torch.float32, device=DEV)
+        L.call("tg_apply32", op, L.ptr(du), None,
+               L.ep(L.TG_EP_CHEB_R, dinv=t(dinv), d=dd, x=dx, r=dr, a1=0.3, a2=0.2, a3=1.0), stream())
+        fm = np.repeat(mask, 3).reshape(-1, 3) >> np.arange(3) & 1
+        fm = fm.ravel().astype(bool)
+        u0 = np.where(fm, u[:n], 0.0)
+        rn = np.where(fm, r0 - ref[:n], 0.0)
+        dn = np.where(fm, 0.3 * dinv * rn + 0.2 * u0, 0.0)
+        assert rel_l2(h(dr), rn) < tol
+        assert rel_l2(h(dd), dn) < tol
+        assert rel_l2(h(dx), np.where(fm, x0 + u0 + dn, 0.0)) < tol
+
+
 @pytest.mark.parametrize("kind", [L.TG_OP_HEX_F32, L.TG_OP_HEX_BF16])
 def test_epilogues(kind):
     dims = (6, 5, 7)

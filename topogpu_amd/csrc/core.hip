@@ -89,19 +89,31 @@ int vec_blocks(long long n) {
   if (b > 4096) b = 4096;
   return (int)b;
 }
+#ifndef TG_RED_BLOCKS
+#define TG_RED_BLOCKS 1024
+#endif
+// Reducing kernels: fewer, fuller blocks, so the one-block finalize sums
+// fewer partials.
+int red_blocks(long long n) {
+  const int b = vec_blocks(n);
+  return b < TG_RED_BLOCKS ? b : TG_RED_BLOCKS;
+}
 
 __global__ void k_copy32(long long n, float* __restrict__ dst, const float* __restrict__ src) {
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
 
 __global__ void k_fill32(long long n, float* x, float v) {
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     x[i] = v;
 }
 __global__ void k_fill64(long long n, double* x, double v) {
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     x[i] = v;
@@ -109,7 +121,9 @@ __global__ void k_fill64(long long n, double* x, double v) {
 
 // Batched vector kernels: blockIdx.y selects the right-hand side; each RHS
 // is a contiguous slice of n entries (slice r starts at r * n). Per-RHS
-// scalars are arrays indexed by r.
+// scalars are arrays indexed by r. The grid-stride loops are unrolled (a
+// thread has about four elements) so that their loads are in flight
+// together.
 __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a, const float* b,
                                                     const float* c, Workspace ws) {
   const long long o = blockIdx.y * n;
@@ -117,6 +131,7 @@ __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a
   b += o;
   if (c) c += o;
   double v[2] = {0.0, 0.0};
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double ai = a[i];
@@ -132,6 +147,7 @@ __global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* 
   a += o;
   b += o;
   double v[1] = {0.0};
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     v[0] = fma(a[i], b[i], v[0]);
@@ -165,6 +181,7 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, flo
   const double den = pq[k];
   const float alpha = (float)(den != 0.0 ? rz[k] / den : 0.0);
   double v[1] = {0.0};
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     x[i] = fmaf(alpha, p[i], x[i]);
@@ -182,6 +199,7 @@ __global__ void k_pcg_dir(long long n, float* p, const float* z, const double* z
   z += o;
   const double den = pq[k];
   const float beta = (float)(den != 0.0 ? -zq[k] / den : 0.0);
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     p[i] = fmaf(beta, p[i], z[i]);
@@ -194,6 +212,7 @@ __global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, fl
   d += o;
   x += o;
   if (r) r += o;
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const float v = a * dinv[i] * rhs[i];
@@ -206,6 +225,7 @@ __global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, fl
 __global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v, const float* kv, const float* dinv,
                                                          Workspace ws) {
   double acc[1] = {0.0};
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double vi = v[i];
@@ -216,6 +236,7 @@ __global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v,
 }
 
 __global__ void k_axpy_f32_to_f64(long long n, double* x, const float* d) {
+  #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     x[i] += (double)d[i];
@@ -344,7 +365,7 @@ extern "C" int tg_dot32_b(int64_t n, int32_t nrhs, const float* a, const float* 
                           double* out2, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && nrhs >= 1 && a && b && out && (c == nullptr) == (out2 == nullptr), "bad dot32 args");
-  const int nb = vec_blocks(n);
+  const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_dot32, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, c,
                      workspace());
   TG_LAUNCH_CHECK();
@@ -357,7 +378,7 @@ extern "C" int tg_dot32(int64_t n, const float* a, const float* b, const float* 
 extern "C" int tg_dot64_b(int64_t n, int32_t nrhs, const double* a, const double* b, double* out, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && nrhs >= 1 && a && b && out, "bad dot64 args");
-  const int nb = vec_blocks(n);
+  const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_dot64, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, a, b, workspace());
   TG_LAUNCH_CHECK();
   return finalize_batched(1, nb, nrhs, out, nullptr, TG_STREAM(s));
@@ -369,7 +390,7 @@ extern "C" int tg_pcg_xr_b(int64_t n, int32_t nrhs, float* x, float* r, const fl
                            const double* rz, const double* pq, double* rr, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && nrhs >= 1 && x && r && p && q && rz && pq && rr, "bad pcg_xr args");
-  const int nb = vec_blocks(n);
+  const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_pcg_xr, dim3(nb, nrhs), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, x, r, p, q, rz, pq,
                      workspace());
   TG_LAUNCH_CHECK();
@@ -414,7 +435,7 @@ extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, fl
 extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv, void* s) {
   TG_NEED_INIT();
   TG_REQUIRE(n > 0 && v && kv && dinv && vdv, "bad power_step args");
-  const int nb = vec_blocks(n);
+  const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_power_step, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
                      workspace());
   TG_LAUNCH_CHECK();

@@ -45,65 +45,96 @@ __host__ __device__ __forceinline__ double axis_w(int i, int I, int nc) {
 
 // Transfers of a batch of right-hand sides: blockIdx.y selects the RHS, whose
 // vectors are contiguous slices of 3 * nodes of their grid.
-__global__ void k_prolong(Grid f, Grid c, const uint8_t* __restrict__ mask_f, const float* __restrict__ ec,
-                          float* __restrict__ xf, int accumulate) {
+// The transfer weights are 1, 1/2 (and their products), exact in FP32.
+// Both kernels evaluate every candidate term, with weight 0 and a clamped
+// (valid) index for a non-parent, so all loads are independent and issue
+// back to back; adding 0 * x leaves a finite sum unchanged, so the result is
+// that of summing the parents alone, in the same order.
+struct Par3 {
+  int idx[3];
+  float w[3];
+};
+
+// Fine nodes 2I-1, 2I, 2I+1 along one axis (nf fine elements, nc coarse)
+// and their prolongation weights from coarse node I.
+__device__ __forceinline__ Par3 restrict_axis(int I, int nf, int nc) {
+  Par3 p;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int i = 2 * I + d - 1;
+    const bool in = (i >= 0) & (i <= nf);
+    p.idx[d] = in ? i : (i < 0 ? 0 : nf);
+    p.w[d] = in ? (float)axis_w(i, I, nc) : 0.0f;
+  }
+  return p;
+}
+
+__device__ __forceinline__ float3 ld3(const float* p) { return make_float3(p[0], p[1], p[2]); }
+
+__global__ void __launch_bounds__(256) k_prolong(Grid f, Grid c, const uint8_t* __restrict__ mask_f,
+                                                 const float* __restrict__ ec, float* __restrict__ xf, int accumulate) {
   ec += (long long)blockIdx.y * 3 * c.nodes();
   xf += (long long)blockIdx.y * 3 * f.nodes();
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= f.nodes()) return;
-  const int k = (int)(n % f.NZ()), j = (int)((n / f.NZ()) % f.NY()), i = (int)(n / ((long long)f.NZ() * f.NY()));
+  const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (unsigned)f.nodes()) return;
+  const unsigned t = n / (unsigned)f.NZ();
+  const int k = (int)(n - t * (unsigned)f.NZ()), j = (int)(t % (unsigned)f.NY()), i = (int)(t / (unsigned)f.NY());
   const AxisW ax = axis_weights(i, c.nx), ay = axis_weights(j, c.ny), az = axis_weights(k, c.nz);
-  float acc[3] = {0.f, 0.f, 0.f};
+  float3 v[8];
+  float w[8];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const int I = (a & 4) ? ax.hi : ax.lo, J = (a & 2) ? ay.hi : ay.lo, K = (a & 1) ? az.hi : az.lo;
-    const double w = ((a & 4) ? ax.whi : ax.wlo) * ((a & 2) ? ay.whi : ay.wlo) * ((a & 1) ? az.whi : az.wlo);
-    if (w == 0.0) continue;
-    const float* p = ec + 3 * c.node(I, J, K);
+    w[a] = (float)(((a & 4) ? ax.whi : ax.wlo) * ((a & 2) ? ay.whi : ay.wlo) * ((a & 1) ? az.whi : az.wlo));
+    v[a] = ld3(ec + 3 * ((I * c.NY() + J) * c.NZ() + K));
+  }
+  float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) acc[q] = fmaf((float)w, p[q], acc[q]);
+  for (int a = 0; a < 8; ++a) {
+    acc[0] = fmaf(w[a], v[a].x, acc[0]);
+    acc[1] = fmaf(w[a], v[a].y, acc[1]);
+    acc[2] = fmaf(w[a], v[a].z, acc[2]);
   }
   const unsigned m = mask_f[n];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const float base = accumulate ? xf[3 * n + q] : 0.0f;
-    xf[3 * n + q] = (m >> q & 1u) ? base + acc[q] : base;
-  }
+  float* o = xf + 3 * (size_t)n;
+  const float3 base = accumulate ? ld3(o) : make_float3(0.f, 0.f, 0.f);
+  o[0] = (m & 1u) ? base.x + acc[0] : base.x;
+  o[1] = (m & 2u) ? base.y + acc[1] : base.y;
+  o[2] = (m & 4u) ? base.z + acc[2] : base.z;
 }
 
-__global__ void k_restrict(Grid f, Grid c, const uint8_t* __restrict__ mask_c, const float* __restrict__ rf,
-                           float* __restrict__ rc) {
+__global__ void __launch_bounds__(256) k_restrict(Grid f, Grid c, const uint8_t* __restrict__ mask_c,
+                                                  const float* __restrict__ rf, float* __restrict__ rc) {
   rf += (long long)blockIdx.y * 3 * f.nodes();
   rc += (long long)blockIdx.y * 3 * c.nodes();
-  const long long N = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (N >= c.nodes()) return;
-  const int K = (int)(N % c.NZ()), J = (int)((N / c.NZ()) % c.NY()), I = (int)(N / ((long long)c.NZ() * c.NY()));
+  const unsigned N = blockIdx.x * blockDim.x + threadIdx.x;
+  if (N >= (unsigned)c.nodes()) return;
+  const unsigned t = N / (unsigned)c.NZ();
+  const int K = (int)(N - t * (unsigned)c.NZ()), J = (int)(t % (unsigned)c.NY()), I = (int)(t / (unsigned)c.NY());
+  const Par3 px = restrict_axis(I, f.nx, c.nx), py = restrict_axis(J, f.ny, c.ny), pz = restrict_axis(K, f.nz, c.nz);
   float acc[3] = {0.f, 0.f, 0.f};
-  for (int di = -1; di <= 1; ++di) {
-    const int i = 2 * I + di;
-    if (i < 0 || i > f.nx) continue;
-    const double wx = axis_w(i, I, c.nx);
-    if (wx == 0.0) continue;
-    for (int dj = -1; dj <= 1; ++dj) {
-      const int j = 2 * J + dj;
-      if (j < 0 || j > f.ny) continue;
-      const double wy = axis_w(j, J, c.ny);
-      if (wy == 0.0) continue;
-      for (int dk = -1; dk <= 1; ++dk) {
-        const int k = 2 * K + dk;
-        if (k < 0 || k > f.nz) continue;
-        const double wz = axis_w(k, K, c.nz);
-        if (wz == 0.0) continue;
-        const float w = (float)(wx * wy * wz);
-        const float* p = rf + 3 * f.node(i, j, k);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, p[q], acc[q]);
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const float wxy = px.w[a] * py.w[b];
+      const float* line = rf + 3 * ((px.idx[a] * f.NY() + py.idx[b]) * f.NZ());
+      float3 v[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = ld3(line + 3 * pz.idx[d]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float w = wxy * pz.w[d];
+        acc[0] = fmaf(w, v[d].x, acc[0]);
+        acc[1] = fmaf(w, v[d].y, acc[1]);
+        acc[2] = fmaf(w, v[d].z, acc[2]);
       }
     }
-  }
   const unsigned m = mask_c[N];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) rc[3 * N + q] = (m >> q & 1u) ? acc[q] : 0.f;
+  float* o = rc + 3 * (size_t)N;
+  o[0] = (m & 1u) ? acc[0] : 0.f;
+  o[1] = (m & 2u) ? acc[1] : 0.f;
+  o[2] = (m & 4u) ? acc[2] : 0.f;
 }
 
 // One block per coarse element: K_C = sum_f (M_f P_f)^T K_f (M_f P_f).
@@ -423,6 +454,8 @@ static int check_pair(const tg_grid* fine, const tg_grid* coarse) {
                  coarse->nely == (fine->nely / 2 > 1 ? fine->nely / 2 : 1) &&
                  coarse->nelz == (fine->nelz / 2 > 1 ? fine->nelz / 2 : 1),
              "coarse grid must be max(1, n/2) of the fine grid");
+  // the transfer kernels index vectors with 32-bit arithmetic
+  TG_REQUIRE(3LL * to_grid(fine).nodes() < (1LL << 31), "grid too large for the transfer kernels");
   return TG_OK;
 }
 

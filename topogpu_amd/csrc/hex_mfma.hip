@@ -50,8 +50,15 @@ extern __device__ uint16_t c_q8_bf16[8 * 2 * 2 * 2 * 64 * 8];
 
 namespace {
 
-constexpr uint32_t kOOB = 0x80000000u;   // offset past every descriptor
 constexpr uint32_t kMaxRange = 0x7fffffffu;
+
+
+// Offset o when valid, else an offset past every descriptor: valid offsets
+// stay below 2^31 (descriptor ranges are at most 2^31 - 1 bytes), so setting
+// bit 31 is enough. Written without a select so that the compiler keeps the
+// loads branch-free; a branch around a load makes it wait for every load in
+// flight before reusing the branch's registers.
+__device__ __forceinline__ uint32_t oob_unless(uint32_t o, bool valid) { return o | ((uint32_t)!valid << 31); }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx8 __attribute__((ext_vector_type(8)));
@@ -67,6 +74,7 @@ struct TileT {
   static constexpr int UBX = MTX + 2, UBY = MTY + 2, UBZ = MTZ + 2;   // node brick
   static constexpr int SZ = 16;                   // z records per slot side (15 used)
   static constexpr int NB = UBX * UBY * UBZ, NIT = (NB + NTHR - 1) / NTHR;
+  static constexpr int NBP = NIT * NTHR;          // brick padded to whole load rounds
   static constexpr int NOUT = MTX * MTY * MTZ, OIT = (NOUT + NTHR - 1) / NTHR;
   static constexpr int SLOT = MTX * MTY * 2 * SZ * 3;
 };
@@ -132,19 +140,15 @@ struct Block {
   BufAcc acc;
 };
 
-// Right-hand side blockIdx.y of a batch: every per-RHS vector is offset by
-// blockIdx.y * 3 * nodes (dinv is the operator's and shared).
+// Geometry and descriptors of tile t; u and ep are those of one right-hand
+// side.
 template <class T>
-__device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const uint8_t* mask, const float* u,
-                                             const Ep32& ep_all) {
+__device__ __forceinline__ Block tile_block(const Grid& g, TileShape ts, int t, const uint8_t* mask, const float* u,
+                                            const Ep32& ep) {
   Block b;
-  const long long roff = (long long)blockIdx.y * 3 * g.nodes();
-  u += roff;
-  const Ep32 ep = ep_slice(ep_all, roff);
   b.NXn = g.nx + 1;
   b.NYn = g.ny + 1;
   b.NZn = g.nz + 1;
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tz = t % ts.ntz, ty = (t / ts.ntz) % ts.nty, tx = t / (ts.ntz * ts.nty);
   b.i0 = tx * T::MTX;
   b.j0 = ty * T::MTY;
@@ -165,6 +169,16 @@ __device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const 
   return b;
 }
 
+// One tile per workgroup. Right-hand side blockIdx.y of a batch: every
+// per-RHS vector is offset by blockIdx.y * 3 * nodes (dinv is the
+// operator's and shared).
+template <class T>
+__device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const uint8_t* mask, const float* u,
+                                             const Ep32& ep_all) {
+  const long long roff = (long long)blockIdx.y * 3 * g.nodes();
+  return tile_block<T>(g, ts, xcd_remap(blockIdx.x, gridDim.x), mask, u + roff, ep_slice(ep_all, roff));
+}
+
 // Node brick loads: every load issued before any is used.
 template <class T>
 __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT][3], uint32_t (&bm)[T::NIT]) {
@@ -173,25 +187,27 @@ __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT
     const int idx = threadIdx.x + it * T::NTHR;
     const int bz = idx % T::UBZ, line = idx / T::UBZ, by = line % T::UBY, bx = line / T::UBY;
     const int i = b.i0 - 1 + bx, j = b.j0 - 1 + by, k = b.k0 - 1 + bz;
-    const bool in = idx < T::NB && (unsigned)i < (unsigned)b.NXn && (unsigned)j < (unsigned)b.NYn &&
-                    (unsigned)k < (unsigned)b.NZn;
+    const bool in = (idx < T::NB) & ((unsigned)i < (unsigned)b.NXn) & ((unsigned)j < (unsigned)b.NYn) &
+                    ((unsigned)k < (unsigned)b.NZn);
     const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
-    const auto w3 = __builtin_amdgcn_raw_buffer_load_b96(b.rs_u, in ? 12u * rel : kOOB, 0, 0);
-    bm[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, in ? rel : kOOB, 0, 0);
+    const auto w3 = __builtin_amdgcn_raw_buffer_load_b96(b.rs_u, oob_unless(12u * rel, in), 0, 0);
+    bm[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, oob_unless(rel, in), 0, 0);
     bw[it][0] = w3[0];
     bw[it][1] = w3[1];
     bw[it][2] = w3[2];
   }
 }
 
-// Masked node values into LDS (fixed DOFs read as zero).
+// Masked node values into LDS (fixed DOFs read as zero). The brick is
+// padded to whole load rounds, so every thread stores unconditionally (its
+// padding entries hold zeros): a conditional store would let the compiler
+// sink the matching loads into the branch and wait for them there.
 template <class T>
 __device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::NIT][3], const uint32_t (&bm)[T::NIT]) {
 #pragma unroll
   for (int it = 0; it < T::NIT; ++it) {
     const int idx = threadIdx.x + it * T::NTHR;
-    if (idx < T::NB)
-      su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
+    su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
                             (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
                             (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
   }
@@ -207,10 +223,10 @@ __device__ __forceinline__ void out_prefetch(Block& b, const Ep32& ep, uint32_t 
     const int idx = threadIdx.x + it * T::NTHR;
     const int z = idx % T::MTZ, line = idx / T::MTZ, ti = line / T::MTY, tj = line % T::MTY;
     const int i = b.i0 + ti, j = b.j0 + tj, k = b.k0 + z;
-    const bool own = idx < T::NOUT && i < b.NXn && j < b.NYn && k < b.NZn;
+    const bool own = (idx < T::NOUT) & (i < b.NXn) & (j < b.NYn) & (k < b.NZn);
     const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
-    ob[it] = own ? 12u * rel : kOOB;
-    om[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, own ? rel : kOOB, 0, 0);
+    ob[it] = oob_unless(12u * rel, own);
+    om[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, oob_unless(rel, own), 0, 0);
     b.acc.b = ob[it];
     EpMode<MODE>::load(ep, b.acc, oin[it]);
   }
@@ -298,54 +314,40 @@ __device__ __forceinline__ void mma2(const bf16x8 (&ah)[2], const bf16x8 (&al)[2
   }
 }
 
-// ---------------------------------------------------------------- level 0
-template <bool SPLIT, int MODE>
-__global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
-                                                         const float* __restrict__ E, const float* __restrict__ u,
-                                                         Ep32 ep, Workspace ws) {
-  using T = Tile0;
-  using Mode = EpMode<MODE>;
-  __shared__ float4 su[T::NB];
-  // [x line][y line][side][z][3]: side 0 is the share of the element column
-  // on the -x side of the line, side 1 the +x side.
-  __shared__ float slot[T::SLOT];
-
-  Block b = block_setup<T>(g, ts, mask, u, ep);
+// Element moduli of wavefront w's pencils (lane element je) in tile b.
+template <class T>
+__device__ __forceinline__ void e_load(const Grid& g, const Block& b, const float* E, int w, int je,
+                                       float (&ev)[T::PY]) {
   const long long eslab = (long long)b.is * g.ny * g.nz, ne = (long long)g.nx * g.ny * g.nz;
   const rsrc_t rs_e = make_rsrc(E + eslab, 4ull * (ne - eslab));
-
-  uint32_t bw[T::NIT][3], bm[T::NIT];
-  brick_load<T>(b, bw, bm);
-  uint32_t ob[T::OIT], om[T::OIT];
-  typename Mode::In oin[T::OIT];
-  out_prefetch<T, MODE>(b, ep, ob, om, oin);
-
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const PencilLane<T> pl(w);
-  // Element moduli of this wavefront's pencils.
-  float ev[T::PY];
-  {
-    const int ci = b.i0 - 1 + w, ck = b.k0 - 1 + pl.je;
-    const bool okx = (unsigned)ci < (unsigned)g.nx && (unsigned)ck < (unsigned)g.nz;
-    const uint32_t ebase = (uint32_t)(((ci - b.is) * g.ny + (b.j0 - 1)) * g.nz + ck);
+  const int ci = b.i0 - 1 + w, ck = b.k0 - 1 + je;
+  const bool okx = ((unsigned)ci < (unsigned)g.nx) & ((unsigned)ck < (unsigned)g.nz);
+  const uint32_t ebase = (uint32_t)(((ci - b.is) * g.ny + (b.j0 - 1)) * g.nz + ck);
 #pragma unroll
-    for (int py = 0; py < T::PY; ++py) {
-      const bool ok = okx && (unsigned)(b.j0 - 1 + py) < (unsigned)g.ny;
-      ev[py] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                             rs_e, ok ? 4u * (ebase + py * g.nz) : kOOB, 0, 0));
-    }
+  for (int py = 0; py < T::PY; ++py) {
+    const bool ok = okx & ((unsigned)(b.j0 - 1 + py) < (unsigned)g.ny);
+    ev[py] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                           rs_e, oob_unless(4u * (ebase + py * g.nz), ok), 0, 0));
   }
-  bf16x8 ah[2][2], al[2][2];
+}
+
+// KE fragments of both output layouts (hi part; lo part in split mode).
+template <bool SPLIT>
+__device__ __forceinline__ void ke_frags(int lane, bf16x8 (&ah)[2][2], bf16x8 (&al)[2][2]) {
 #pragma unroll
   for (int ly = 0; ly < 2; ++ly)
 #pragma unroll
     for (int tl = 0; tl < 2; ++tl) {
-      ah[ly][tl] = load_ke(ly, 0, tl, pl.lane);
-      al[ly][tl] = SPLIT ? load_ke(ly, 1, tl, pl.lane) : ah[ly][tl];
+      ah[ly][tl] = load_ke(ly, 0, tl, lane);
+      al[ly][tl] = SPLIT ? load_ke(ly, 1, tl, lane) : ah[ly][tl];
     }
-  brick_store<T>(su, bw, bm);
-  __syncthreads();
+}
 
+// The wavefront's PY pencils: element products from the brick in LDS, z and
+// y reductions in registers, x shares into the LDS slots.
+template <bool SPLIT, class T>
+__device__ __forceinline__ void pencils0(const float4* su, float* slot, const PencilLane<T>& pl,
+                                         const bf16x8 (&ah)[2][2], const bf16x8 (&al)[2][2], const float (&ev)[T::PY]) {
   float rp[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int py = 0; py < T::PY; ++py) {
@@ -364,6 +366,36 @@ __global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, 
 #pragma unroll
     for (int v = 0; v < 3; ++v) rp[v] = rc[v];
   }
+}
+
+// ---------------------------------------------------------------- level 0
+template <bool SPLIT, int MODE>
+__global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
+                                                         const float* __restrict__ E, const float* __restrict__ u,
+                                                         Ep32 ep, Workspace ws) {
+  using T = Tile0;
+  using Mode = EpMode<MODE>;
+  __shared__ float4 su[T::NBP];
+  // [x line][y line][side][z][3]: side 0 is the share of the element column
+  // on the -x side of the line, side 1 the +x side.
+  __shared__ float slot[T::SLOT];
+
+  Block b = block_setup<T>(g, ts, mask, u, ep);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const PencilLane<T> pl(w);
+  // loads in the order they are needed (they complete in issue order)
+  bf16x8 ah[2][2], al[2][2];
+  ke_frags<SPLIT>(pl.lane, ah, al);
+  uint32_t bw[T::NIT][3], bm[T::NIT];
+  brick_load<T>(b, bw, bm);
+  float ev[T::PY];
+  e_load<T>(g, b, E, w, pl.je, ev);
+  uint32_t ob[T::OIT], om[T::OIT];
+  typename Mode::In oin[T::OIT];
+  out_prefetch<T, MODE>(b, ep, ob, om, oin);
+  brick_store<T>(su, bw, bm);
+  __syncthreads();
+  pencils0<SPLIT, T>(su, slot, pl, ah, al, ev);
   __syncthreads();
 
   const double part = out_store<T, MODE>(b, ep, slot, su, ob, om, oin);
@@ -387,7 +419,7 @@ __global__ void __launch_bounds__(Tile1::NTHR) k_q8_mfma(Grid g, TileShape ts, c
   using T = Tile1;
   using Mode = EpMode<MODE>;
   constexpr int H0 = (T::PY + 1) / 2;   // pencils of the first half
-  __shared__ float4 su[T::NB];
+  __shared__ float4 su[T::NBP];
   __shared__ float slot[T::SLOT];
 
   Block b = block_setup<T>(g, ts, mask, u, ep);
@@ -405,12 +437,12 @@ __global__ void __launch_bounds__(Tile1::NTHR) k_q8_mfma(Grid g, TileShape ts, c
   const PencilLane<T> pl(w);
   // fine offset of child (0,0,0) of the lane's element in pencil py = 0
   const int ci = b.i0 - 1 + w, ck = b.k0 - 1 + pl.je;
-  const bool okx = (unsigned)ci < (unsigned)g.nx && (unsigned)ck < (unsigned)g.nz;
+  const bool okx = ((unsigned)ci < (unsigned)g.nx) & ((unsigned)ck < (unsigned)g.nz);
   const uint32_t fbase = (uint32_t)(((2 * (ci - b.is)) * fny + 2 * (b.j0 - 1)) * fnz + 2 * ck);
   auto child_e = [&](int s, int py) -> float {
-    const bool ok = okx && (unsigned)(b.j0 - 1 + py) < (unsigned)g.ny;
+    const bool ok = okx & ((unsigned)(b.j0 - 1 + py) < (unsigned)g.ny);
     const uint32_t off = fbase + (uint32_t)(((s >> 2) * fny + 2 * py + ((s >> 1) & 1)) * fnz + (s & 1));
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_e, ok ? 4u * off : kOOB, 0, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_e, oob_unless(4u * off, ok), 0, 0));
   };
   brick_store<T>(su, bw, bm);
   __syncthreads();

@@ -69,7 +69,7 @@ def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: 
 class GMG:
     def __init__(self, dims, mask0: np.ndarray, *, n_levels: int = 4, fine_op: int = L.TG_OP_HEX_BF16,
                  fine_accurate_op: int | None = None, l1_q8_op: int = L.TG_OP_Q8_F32,
-                 l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree: int = 2,
+                 l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree=2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
                  nrhs: int = 1):
@@ -80,7 +80,12 @@ class GMG:
         if not 1 <= self.nrhs <= 64:
             raise ValueError("nrhs must be 1..64")
         self.fine_degree = max(1, int(fine_degree))
-        self.coarse_degree = max(1, int(coarse_degree))
+        # an int, or one degree per level from level 1 down (the last entry
+        # repeats for deeper levels)
+        cd = [coarse_degree] if np.isscalar(coarse_degree) else list(coarse_degree)
+        if not cd:
+            raise ValueError("coarse_degree must not be empty")
+        self.coarse_degrees = [max(1, int(d)) for d in cd]
         self.frac = float(cheb_lower_frac)
         self.safety = float(cheb_safety)
         self.power_iters = power_iters
@@ -289,7 +294,7 @@ class GMG:
             L.call("tg_coarsest_apply_b", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
                    self.nrhs, L.ptr(rhs), L.ptr(Lv.x), stream)
             return Lv.x
-        deg = self.fine_degree if lv == 0 else self.coarse_degree
+        deg = self.fine_degree if lv == 0 else self.coarse_degrees[min(lv, len(self.coarse_degrees)) - 1]
         self._presmooth(lv, rhs, deg, stream)
         C = self.levels[lv + 1]
         L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
