@@ -34,7 +34,7 @@ enum {
   TG_ERR_ARG = 1,     /* invalid argument or unsupported shape            */
   TG_ERR_HIP = 2,     /* HIP runtime / kernel launch failure              */
   TG_ERR_STATE = 3,   /* tg_init not called / library state missing       */
-  TG_ERR_NOT_SPD = 4  /* dense Cholesky failed                            */
+  TG_ERR_NOT_SPD = 4  /* dense SPD inverse met a non-positive pivot      */
 };
 
 typedef struct {
@@ -196,10 +196,13 @@ int tg_galerkin_from_fine(const tg_grid* fine, int levels_down, const double* fi
 int tg_assemble_dense(const tg_grid* g, const double* mats64, const int32_t* free_index,
                       int32_t n_free, double* A, void* stream);
 
-/* In-place dense FP64 Cholesky of A (n x n, row-major SPD) followed by the
- * explicit inverse A^-1 (full symmetric) written to A. Host-synchronous.
- * Replaces cp.linalg.cholesky + solve_triangular (multigrid_v4.py:781-801,
- * 977-978). Returns TG_ERR_NOT_SPD when the factorisation fails. */
+/* In-place explicit inverse A^-1 (full symmetric) of a dense FP64 SPD
+ * matrix A (n x n, row-major), by Gauss-Jordan elimination without pivoting:
+ * one workgroup for n <= 64, 16-wide pivot blocks with tiled updates above.
+ * Host-synchronous. Replaces cp.linalg.cholesky + solve_triangular
+ * (multigrid_v4.py:781-801, 977-978): the V-cycle then applies the coarsest
+ * solve as one matrix-vector product. Returns TG_ERR_NOT_SPD when a pivot is
+ * not positive. */
 int tg_dense_spd_inverse(double* A, int32_t n, void* stream);
 
 /* ec = scatter(Ainv * gather(rc)) over the free DOFs of the coarsest level;

@@ -119,11 +119,11 @@ def test_rigid_translation_is_exact_zero(prec):
 
 @pytest.mark.parametrize("dims", [(128, 64, 64), (141, 33, 95)])
 def test_hex_mfma_grids_larger_than_one_wave_of_workgroups(dims):
-    """Grids with more tiles than workgroups resident at once (the level-0
-    MFMA operator then runs as a persistent, pipelined kernel): every tile
-    is computed once, against the FP64 device operator (itself checked
-    against the oracle above), for STORE, DOT and CHEB_R, one and two
-    right-hand sides."""
+    """Grids with more tiles than workgroups resident at once (several
+    rounds of workgroups, tiles cut at every grid face): every tile is
+    computed once, against the FP64 device operator (itself checked against
+    the oracle above), for STORE, DOT and CHEB_R, one and two right-hand
+    sides."""
     rng = np.random.default_rng(31)
     nn = int(np.prod([d + 1 for d in dims]))
     n = 3 * nn
@@ -375,7 +375,7 @@ def test_galerkin_levels_match_ptkp(dims, point):
         assert rel_inf(h(d32)[fr], 1.0 / Kc.diagonal()) < 1e-6
         fine_m = m64
         if lv == 2:
-            # dense coarsest: assembly, Cholesky inverse, application
+            # dense coarsest: assembly, explicit inverse, application
             fidx = -np.ones(nd, np.int32)
             fidx[fr] = np.arange(fr.size, dtype=np.int32)
             A = torch.zeros(fr.size * fr.size, dtype=torch.float64, device=DEV)
@@ -426,17 +426,18 @@ def test_not_spd_is_reported():
     A = t(np.array([1.0, 2.0, 2.0, 1.0]))
     with pytest.raises(L.TopoHipError, match="not SPD"):
         L.call("tg_dense_spd_inverse", L.ptr(A), 2, stream())
-    n = 400   # library path
-    B = np.eye(n)
-    B[n - 1, n - 1] = -1.0
-    with pytest.raises(L.TopoHipError, match="not SPD"):
-        L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), n, stream())
+    for n, bad in ((400, 399), (200, 100), (130, 0)):   # blocked path: last, middle, first block
+        B = np.eye(n)
+        B[bad, bad] = -1.0
+        with pytest.raises(L.TopoHipError, match=f"not SPD \\(pivot {bad + 1}\\)"):
+            L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), n, stream())
 
 
-@pytest.mark.parametrize("n", [1, 7, 135, 192, 193, 600])
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 135, 193, 600, 1080])
 def test_dense_spd_inverse_sizes(n):
-    """One-workgroup Gauss-Jordan (n <= 192) and library Cholesky (n > 192)
-    invert an ill-conditioned SPD matrix to FP64 accuracy, symmetric."""
+    """One-workgroup Gauss-Jordan (n <= 64) and blocked Gauss-Jordan (64 x 64
+    blocks, n > 64) invert an ill-conditioned SPD matrix to FP64 accuracy,
+    symmetric."""
     rng = np.random.default_rng(n)
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     ev = np.logspace(0, 8, n)   # condition number 1e8

@@ -8,7 +8,6 @@
 // solver_v2.py:62-149). P is the Kronecker product of the three axes, so the
 // normalisation factorises per axis.
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "common.h"
 #include "reduce.h"
@@ -254,30 +253,48 @@ __global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const i
   if (lane == 0) ec[fl[row]] = (float)s;
 }
 
-// In-place inverse of a symmetric positive definite n x n matrix (row-major)
-// by Gauss-Jordan elimination without pivoting, in one workgroup. The pivots
-// of Gauss-Jordan on an SPD matrix are the pivots of its Cholesky
-// factorisation squared, so they stay positive; a pivot <= 0 (or NaN)
-// reports "not SPD" through info.
-// The matrix lives in registers: thread (tr, tc) = (tid / 32, tid % 32) owns
-// the entries (tr + 32a, tc + 32b), a, b < GJ_NB. Step k publishes row k and
-// column k through LDS and every thread updates its own entries, so each of
-// the n steps costs two barriers and a few FMAs per thread, with no
-// global-memory round trip.
-constexpr int GJ_THREADS = 1024, GJ_NB = 6;
-constexpr int GJ_MAXN = 32 * GJ_NB;   // 192
+// Inverse of a symmetric positive definite n x n matrix (row-major) by
+// Gauss-Jordan elimination without pivoting. The pivots of Gauss-Jordan on
+// an SPD matrix are the pivots of its Cholesky factorisation squared, so
+// they stay positive; a pivot <= 0 (or NaN) reports "not SPD" through info.
+//
+// Small matrices (n <= 64), and the diagonal blocks of the blocked form, are
+// inverted by one workgroup with the matrix in registers: thread (tr, tc) =
+// (tid / 32, tid % 32) owns the entries (tr + 32a, tc + 32b), a, b < NB.
+// Step k publishes row k and column k through LDS and every thread updates
+// its own entries, so each of the n steps costs two barriers and a few FMAs
+// per thread, with no global-memory round trip.
+//
+// Larger matrices use the blocked form with BK-wide pivot blocks: for each
+// diagonal block k, with P = inverse of the current A_kk,
+//   A_kj <- P A_kj,  A_ij <- A_ij - A_ik A_kj (new A_kj),  A_ik <- -A_ik P,
+//   A_kk <- P      (i, j outside block k),
+// which leaves the inverse in place after the last block. The updates are
+// tiled FP64 kernels over TS x TS tiles; each block step is four launches.
+// The pivot block stays narrow: the panels are multiplied by an explicit
+// block inverse, whose rounding on an ill-conditioned matrix grows with the
+// block width (at condition 1e8, 64-wide blocks lose ~1e-5 relative
+// accuracy, 16-wide blocks stay at the ~1e-9 of scalar elimination).
+constexpr int GJ_THREADS = 1024;
+constexpr int BK = 16, TS = 64;
 
-__global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(double* A, int n, int* info) {
-  __shared__ double prow[GJ_MAXN];
-  __shared__ double pcol[GJ_MAXN];
+// Invert the b x b matrix at A (leading dimension lda) into P (leading
+// dimension ldp; P may be A). info: first failing pivot + base + 1; with
+// zero_info, thread 0 clears *info first.
+template <int NB>
+__global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(const double* A, int lda, int n, double* P, int ldp,
+                                                             int* info, int info_base, int zero_info) {
+  __shared__ double prow[32 * NB];
+  __shared__ double pcol[32 * NB];
   const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
-  double a[GJ_NB][GJ_NB];
+  if (zero_info && tid == 0) *info = 0;
+  double a[NB][NB];
 #pragma unroll
-  for (int x = 0; x < GJ_NB; ++x)
+  for (int x = 0; x < NB; ++x)
 #pragma unroll
-    for (int y = 0; y < GJ_NB; ++y) {
+    for (int y = 0; y < NB; ++y) {
       const int i = tr + 32 * x, j = tc + 32 * y;
-      a[x][y] = (i < n && j < n) ? A[(long long)i * n + j] : 0.0;
+      a[x][y] = (i < n && j < n) ? A[(long long)i * lda + j] : 0.0;
     }
   for (int k = 0; k < n; ++k) {
     const int kq = k >> 5, kr = k & 31;
@@ -285,52 +302,145 @@ __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(double* A, int n,
     // register indexing (which would move the tile to scratch memory)
     if (tc == kr)
 #pragma unroll
-      for (int x = 0; x < GJ_NB; ++x) {
+      for (int x = 0; x < NB; ++x) {
         double v = 0.0;
 #pragma unroll
-        for (int y = 0; y < GJ_NB; ++y) v = (y == kq) ? a[x][y] : v;
+        for (int y = 0; y < NB; ++y) v = (y == kq) ? a[x][y] : v;
         pcol[tr + 32 * x] = v;
       }
     if (tr == kr)
 #pragma unroll
-      for (int y = 0; y < GJ_NB; ++y) {
+      for (int y = 0; y < NB; ++y) {
         double v = 0.0;
 #pragma unroll
-        for (int x = 0; x < GJ_NB; ++x) v = (x == kq) ? a[x][y] : v;
+        for (int x = 0; x < NB; ++x) v = (x == kq) ? a[x][y] : v;
         prow[tc + 32 * y] = v;
       }
     __syncthreads();
     const double p = pcol[k];   // uniform
     if (!(p > 0.0)) {
-      if (tid == 0) *info = k + 1;
+      if (tid == 0) *info = info_base + k + 1;
       return;
     }
     const double ip = 1.0 / p;
-    double pr[GJ_NB], pc[GJ_NB];
+    double pr[NB], pc[NB];
 #pragma unroll
-    for (int y = 0; y < GJ_NB; ++y) {
+    for (int y = 0; y < NB; ++y) {
       const int j = tc + 32 * y;
       pr[y] = (j == k ? 1.0 : prow[j]) * ip;
     }
 #pragma unroll
-    for (int x = 0; x < GJ_NB; ++x) pc[x] = pcol[tr + 32 * x];
+    for (int x = 0; x < NB; ++x) pc[x] = pcol[tr + 32 * x];
 #pragma unroll
-    for (int x = 0; x < GJ_NB; ++x)
+    for (int x = 0; x < NB; ++x)
 #pragma unroll
-      for (int y = 0; y < GJ_NB; ++y) {
+      for (int y = 0; y < NB; ++y) {
         const int i = tr + 32 * x, j = tc + 32 * y;
         a[x][y] = (i == k) ? pr[y] : (j == k ? 0.0 : a[x][y]) - pc[x] * pr[y];
       }
     __syncthreads();
   }
 #pragma unroll
-  for (int x = 0; x < GJ_NB; ++x)
+  for (int x = 0; x < NB; ++x)
 #pragma unroll
-    for (int y = 0; y < GJ_NB; ++y) {
+    for (int y = 0; y < NB; ++y) {
       const int i = tr + 32 * x, j = tc + 32 * y;
-      if (i < n && j < n) A[(long long)i * n + j] = a[x][y];
+      if (i < n && j < n) P[(long long)i * ldp + j] = a[x][y];
     }
-  if (tid == 0) *info = 0;
+}
+
+// A_kj <- P A_kj: rows [k0, k0 + kw) of column tile blockIdx.x (columns
+// inside the pivot block are left alone).
+__global__ void __launch_bounds__(256) k_gj_row_panel(double* A, int n, int k0, int kw, const double* P) {
+  __shared__ double sp[BK][BK + 1], sb[BK][TS + 1];
+  const int j0 = blockIdx.x * TS, jw = min(TS, n - j0);
+  for (int e = threadIdx.x; e < BK * BK; e += blockDim.x) {
+    const int r = e / BK, c = e % BK;
+    sp[r][c] = (r < kw && c < kw) ? P[r * BK + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < BK * TS; e += blockDim.x) {
+    const int r = e / TS, c = e % TS;
+    sb[r][c] = (r < kw && c < jw) ? A[(long long)(k0 + r) * n + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int c = threadIdx.x % TS, rg = threadIdx.x / TS;   // 4 row groups
+  const int j = j0 + c;
+  const bool colok = c < jw && (j < k0 || j >= k0 + kw);
+#pragma unroll
+  for (int q = 0; q < BK / 4; ++q) {
+    const int r = rg + 4 * q;
+    double acc = 0.0;
+    for (int kk = 0; kk < kw; ++kk) acc = fma(sp[r][kk], sb[kk][c], acc);
+    if (colok && r < kw) A[(long long)(k0 + r) * n + j] = acc;
+  }
+}
+
+// A_ij -= A_ik A_kj on tile (blockIdx.y, blockIdx.x), rows and columns
+// inside the pivot block excluded. Thread (tr, tc) owns rows tr + 16a and
+// columns tc + 16b of the tile, a, b < 4.
+__global__ void __launch_bounds__(256) k_gj_trailing(double* A, int n, int k0, int kw) {
+  __shared__ double sa[TS][BK + 1], sb[BK][TS + 1];
+  const int i0 = blockIdx.y * TS, iw = min(TS, n - i0), j0 = blockIdx.x * TS, jw = min(TS, n - j0);
+  for (int e = threadIdx.x; e < TS * BK; e += blockDim.x) {
+    const int r = e / BK, c = e % BK;
+    sa[r][c] = (r < iw && c < kw) ? A[(long long)(i0 + r) * n + k0 + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < BK * TS; e += blockDim.x) {
+    const int r = e / TS, c = e % TS;
+    sb[r][c] = (r < kw && c < jw) ? A[(long long)(k0 + r) * n + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4] = {};
+  for (int kk = 0; kk < kw; ++kk) {
+    double l[4], r[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) l[a] = sa[tr + 16 * a][kk];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) r[b] = sb[kk][tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(l[a], r[b], acc[a][b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + tr + 16 * a;
+    const bool rowok = tr + 16 * a < iw && (i < k0 || i >= k0 + kw);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tc + 16 * b;
+      if (rowok && tc + 16 * b < jw && (j < k0 || j >= k0 + kw)) A[(long long)i * n + j] -= acc[a][b];
+    }
+  }
+}
+
+// A_ik <- -A_ik P for the rows of row tile blockIdx.x outside the pivot
+// block; the pivot block's rows get A_kk <- P.
+__global__ void __launch_bounds__(256) k_gj_col_panel(double* A, int n, int k0, int kw, const double* P) {
+  __shared__ double sa[TS][BK + 1], sp[BK][BK + 1];
+  const int i0 = blockIdx.x * TS, iw = min(TS, n - i0);
+  for (int e = threadIdx.x; e < BK * BK; e += blockDim.x) {
+    const int r = e / BK, c = e % BK;
+    sp[r][c] = (r < kw && c < kw) ? P[r * BK + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < TS * BK; e += blockDim.x) {
+    const int r = e / BK, c = e % BK;
+    sa[r][c] = (r < iw && c < kw) ? A[(long long)(i0 + r) * n + k0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int r = threadIdx.x % TS, cg = threadIdx.x / TS;   // 4 column groups
+  const int i = i0 + r;
+  if (r >= iw) return;
+  const bool inblock = i >= k0 && i < k0 + kw;
+#pragma unroll
+  for (int q = 0; q < BK / 4; ++q) {
+    const int c = cg + 4 * q;
+    if (c >= kw) continue;
+    double acc = 0.0;
+    for (int kk = 0; kk < kw; ++kk) acc = fma(sa[r][kk], sp[kk][c], acc);
+    A[(long long)i * n + k0 + c] = inblock ? sp[i - k0][c] : -acc;
+  }
 }
 
 struct SolverHandles {
@@ -516,48 +626,57 @@ extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const in
   return TG_OK;
 }
 
+int dense_inverse_blocked(double* A, int n, hipStream_t st);
+
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
-  if (n <= GJ_MAXN) {
-    TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device");
-    int* info_d = workspace().flags;
-    hipLaunchKernelGGL(k_spd_inverse_gj, dim3(1), dim3(GJ_THREADS), 0, (hipStream_t)s, A, n, info_d);
+  TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device");
+  const hipStream_t st = (hipStream_t)s;
+  int* info_d = workspace().flags;
+  if (n <= 64) {
+    if (n <= 32)
+      hipLaunchKernelGGL((k_spd_inverse_gj<1>), dim3(1), dim3(GJ_THREADS), 0, st, A, n, n, A, n, info_d, 0, 1);
+    else
+      hipLaunchKernelGGL((k_spd_inverse_gj<2>), dim3(1), dim3(GJ_THREADS), 0, st, A, n, n, A, n, info_d, 0, 1);
     TG_LAUNCH_CHECK();
-    int info = 0;
-    TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
-    TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
-    if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
-    const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
-    hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)s,
-                       A, n);
+  } else {
+    return dense_inverse_blocked(A, n, st);
+  }
+  int info = 0;
+  TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
+  TG_HIP_CHECK(hipStreamSynchronize(st));
+  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
+  const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+// Blocked Gauss-Jordan with BK-wide pivot blocks; P (the inverted pivot
+// block) lives in the reduction workspace, which no other kernel uses
+// meanwhile on this stream.
+int dense_inverse_blocked(double* A, int n, hipStream_t st) {
+  int* info_d = workspace().flags;
+  double* P = workspace().partials;
+  const int nt = (n + TS - 1) / TS;
+  for (int k0 = 0; k0 < n; k0 += BK) {
+    const int kw = n - k0 < BK ? n - k0 : BK;
+    hipLaunchKernelGGL((k_spd_inverse_gj<1>), dim3(1), dim3(GJ_THREADS), 0, st, A + (long long)k0 * n + k0, n, kw, P,
+                       BK, info_d, k0, k0 == 0 ? 1 : 0);
     TG_LAUNCH_CHECK();
-    return TG_OK;
+    hipLaunchKernelGGL(k_gj_row_panel, dim3(nt), dim3(256), 0, st, A, n, k0, kw, P);
+    TG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gj_trailing, dim3(nt, nt), dim3(256), 0, st, A, n, k0, kw);
+    TG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gj_col_panel, dim3(nt), dim3(256), 0, st, A, n, k0, kw, P);
+    TG_LAUNCH_CHECK();
   }
-  rocblas_handle h = nullptr;
-  if (int r = get_handle((hipStream_t)s, &h)) return r;
-  rocblas_int* dinfo = nullptr;
-  TG_HIP_CHECK(hipMallocAsync((void**)&dinfo, sizeof(rocblas_int), (hipStream_t)s));
-  // Row-major symmetric A is its own column-major transpose; work on the
-  // column-major lower triangle, i.e. the row-major upper triangle.
-  rocblas_int info = 0;
-  rocblas_status st = rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, dinfo);
-  if (st == rocblas_status_success) {
-    TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
-    TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
-  }
-  if (st == rocblas_status_success && info == 0) {
-    st = rocsolver_dpotri(h, rocblas_fill_lower, n, A, n, dinfo);
-    if (st == rocblas_status_success) {
-      TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, (hipStream_t)s));
-      TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
-    }
-  }
-  TG_HIP_CHECK(hipFreeAsync(dinfo, (hipStream_t)s));
-  if (st != rocblas_status_success) return set_error(TG_ERR_HIP, "rocsolver potrf/potri status %d", (int)st);
-  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (Cholesky info=%d)", (int)info);
-  const long long nn = (long long)n * n;
-  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)s, A,
-                     n);
+  int info = 0;
+  TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
+  TG_HIP_CHECK(hipStreamSynchronize(st));
+  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
+  const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

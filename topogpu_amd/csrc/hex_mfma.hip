@@ -73,10 +73,22 @@ struct TileT {
   static constexpr int PY = MTY + 1;              // pencils per wavefront
   static constexpr int UBX = MTX + 2, UBY = MTY + 2, UBZ = MTZ + 2;   // node brick
   static constexpr int SZ = 16;                   // z records per slot side (15 used)
-  static constexpr int NB = UBX * UBY * UBZ, NIT = (NB + NTHR - 1) / NTHR;
-  static constexpr int NBP = NIT * NTHR;          // brick padded to whole load rounds
-  static constexpr int NOUT = MTX * MTY * MTZ, OIT = (NOUT + NTHR - 1) / NTHR;
+  static constexpr int NB = UBX * UBY * UBZ;
+  // Brick loads: thread t < LPR * UBZ takes z = t % UBZ of brick line t / UBZ
+  // and, in round r, the same (y, z) BXR x-planes further on. LPR is a
+  // multiple of UBY, so a round advances x only and the LDS index of a
+  // thread's node is t + r * LPR * UBZ: no per-round index arithmetic.
+  static constexpr int LPR = ((NTHR / UBZ) / UBY) * UBY, BXR = LPR / UBY;
+  static constexpr int NIT = (UBX + BXR - 1) / BXR;   // load rounds
+  static constexpr int BACT = LPR * UBZ;               // threads with brick work
+  static constexpr int NBP = NIT * BACT + (NTHR - BACT);   // + a dump slot per idle thread
+  // Output nodes, the same way: thread t < OLPR * MTZ takes z = t % MTZ of
+  // owned line t / MTZ and, in round r, the line OXR x-lines further on.
+  static constexpr int OLPR = ((NTHR / MTZ) / MTY) * MTY, OXR = OLPR / MTY;
+  static constexpr int OIT = (MTX + OXR - 1) / OXR;    // output rounds
+  static constexpr int OACT = OLPR * MTZ;
   static constexpr int SLOT = MTX * MTY * 2 * SZ * 3;
+  static_assert(LPR >= UBY && OLPR >= MTY, "tile too large for the workgroup");
 };
 using Tile0 = TileT<8>;   // level 0
 using Tile1 = TileT<4>;   // level 1 (8x fewer elements: smaller tiles keep the CUs busy)
@@ -128,8 +140,13 @@ struct BufAcc {
 };
 
 struct TileShape {
-  int ntz, nty;   // tiles along z and y
+  int ntz, nty;     // tiles along z and y
+  uint32_t mz, my;  // ceil(2^32 / ntz), ceil(2^32 / nty) (0 for a divisor of 1)
 };
+
+// q = t / d for the tile counts: umulhi(t, ceil(2^32 / d)) is exact while
+// t * d < 2^32, which the launcher checks.
+__device__ __forceinline__ uint32_t div_tiles(uint32_t t, uint32_t m) { return m ? __umulhi(t, m) : t; }
 
 // Workgroup geometry and node-vector descriptors.
 struct Block {
@@ -149,7 +166,8 @@ __device__ __forceinline__ Block tile_block(const Grid& g, TileShape ts, int t, 
   b.NXn = g.nx + 1;
   b.NYn = g.ny + 1;
   b.NZn = g.nz + 1;
-  const int tz = t % ts.ntz, ty = (t / ts.ntz) % ts.nty, tx = t / (ts.ntz * ts.nty);
+  const uint32_t q1 = div_tiles((uint32_t)t, ts.mz), q2 = div_tiles(q1, ts.my);
+  const int tz = t - (int)q1 * ts.ntz, ty = (int)q1 - (int)q2 * ts.nty, tx = (int)q2;
   b.i0 = tx * T::MTX;
   b.j0 = ty * T::MTY;
   b.k0 = tz * T::MTZ;
@@ -182,14 +200,15 @@ __device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const 
 // Node brick loads: every load issued before any is used.
 template <class T>
 __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT][3], uint32_t (&bm)[T::NIT]) {
+  const int t = threadIdx.x, bz = t % T::UBZ, l0 = t / T::UBZ, bx0 = l0 / T::UBY, by = l0 % T::UBY;
+  const int i0 = b.i0 - 1 + bx0, j = b.j0 - 1 + by, k = b.k0 - 1 + bz;
+  const bool ok = (t < T::BACT) & ((unsigned)j < (unsigned)b.NYn) & ((unsigned)k < (unsigned)b.NZn);
+  const uint32_t rel0 = (uint32_t)(((i0 - b.is) * b.NYn + j) * b.NZn + k);
+  const uint32_t plane = (uint32_t)(T::BXR * b.NYn * b.NZn);
 #pragma unroll
   for (int it = 0; it < T::NIT; ++it) {
-    const int idx = threadIdx.x + it * T::NTHR;
-    const int bz = idx % T::UBZ, line = idx / T::UBZ, by = line % T::UBY, bx = line / T::UBY;
-    const int i = b.i0 - 1 + bx, j = b.j0 - 1 + by, k = b.k0 - 1 + bz;
-    const bool in = (idx < T::NB) & ((unsigned)i < (unsigned)b.NXn) & ((unsigned)j < (unsigned)b.NYn) &
-                    ((unsigned)k < (unsigned)b.NZn);
-    const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
+    const bool in = ok & ((unsigned)(i0 + it * T::BXR) < (unsigned)b.NXn);
+    const uint32_t rel = rel0 + it * plane;
     const auto w3 = __builtin_amdgcn_raw_buffer_load_b96(b.rs_u, oob_unless(12u * rel, in), 0, 0);
     bm[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, oob_unless(rel, in), 0, 0);
     bw[it][0] = w3[0];
@@ -198,18 +217,20 @@ __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT
   }
 }
 
-// Masked node values into LDS (fixed DOFs read as zero). The brick is
-// padded to whole load rounds, so every thread stores unconditionally (its
-// padding entries hold zeros): a conditional store would let the compiler
-// sink the matching loads into the branch and wait for them there.
+// Masked node values into LDS (fixed DOFs and nodes outside the grid read
+// as zero). Every thread stores unconditionally -- idle threads into a dump
+// slot of their own past the brick -- because a conditional store would let
+// the compiler sink the matching loads into the branch and wait there.
 template <class T>
 __device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::NIT][3], const uint32_t (&bm)[T::NIT]) {
+  const int t = threadIdx.x;
+  const bool act = t < T::BACT;
 #pragma unroll
   for (int it = 0; it < T::NIT; ++it) {
-    const int idx = threadIdx.x + it * T::NTHR;
+    const int idx = act ? t + it * T::BACT : T::NIT * T::BACT + (t - T::BACT);
     su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
-                            (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
-                            (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
+                          (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
+                          (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
   }
 }
 
@@ -218,13 +239,15 @@ __device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::
 template <class T, int MODE>
 __device__ __forceinline__ void out_prefetch(Block& b, const Ep32& ep, uint32_t (&ob)[T::OIT], uint32_t (&om)[T::OIT],
                                              typename EpMode<MODE>::In (&oin)[T::OIT]) {
+  const int t = threadIdx.x, z = t % T::MTZ, l0 = t / T::MTZ, ti0 = l0 / T::MTY, tj = l0 % T::MTY;
+  const int i0 = b.i0 + ti0, j = b.j0 + tj, k = b.k0 + z;
+  const bool ok = (t < T::OACT) & (j < b.NYn) & (k < b.NZn);
+  const uint32_t rel0 = (uint32_t)(((i0 - b.is) * b.NYn + j) * b.NZn + k);
+  const uint32_t plane = (uint32_t)(T::OXR * b.NYn * b.NZn);
 #pragma unroll
   for (int it = 0; it < T::OIT; ++it) {
-    const int idx = threadIdx.x + it * T::NTHR;
-    const int z = idx % T::MTZ, line = idx / T::MTZ, ti = line / T::MTY, tj = line % T::MTY;
-    const int i = b.i0 + ti, j = b.j0 + tj, k = b.k0 + z;
-    const bool own = (idx < T::NOUT) & (i < b.NXn) & (j < b.NYn) & (k < b.NZn);
-    const uint32_t rel = (uint32_t)(((i - b.is) * b.NYn + j) * b.NZn + k);
+    const bool own = ok & (ti0 + it * T::OXR < T::MTX) & (i0 + it * T::OXR < b.NXn);
+    const uint32_t rel = rel0 + it * plane;
     ob[it] = oob_unless(12u * rel, own);
     om[it] = __builtin_amdgcn_raw_buffer_load_b8(b.rs_m, oob_unless(rel, own), 0, 0);
     b.acc.b = ob[it];
@@ -238,17 +261,19 @@ template <class T, int MODE>
 __device__ __forceinline__ double out_store(Block& b, const Ep32& ep, const float* slot, const float4* su,
                                             const uint32_t (&ob)[T::OIT], const uint32_t (&om)[T::OIT],
                                             const typename EpMode<MODE>::In (&oin)[T::OIT]) {
+  // idle threads and rounds past the tile read a valid slot; their stores are dropped
+  const int t = threadIdx.x < T::OACT ? threadIdx.x : 0;
+  const int z = t % T::MTZ, l0 = t / T::MTZ, ti0 = l0 / T::MTY, tj = l0 % T::MTY;
   double part = 0.0;
 #pragma unroll
   for (int it = 0; it < T::OIT; ++it) {
-    int idx = threadIdx.x + it * T::NTHR;
-    idx = idx < T::NOUT ? idx : T::NOUT - 1;   // surplus threads read a valid slot; their stores are dropped
-    const int z = idx % T::MTZ, line = idx / T::MTZ, tli = line / T::MTY, tlj = line % T::MTY;
+    const int tli = ti0 + it * T::OXR < T::MTX ? ti0 + it * T::OXR : T::MTX - 1;
+    const int line = tli * T::MTY + tj;
     const float* s0 = &slot[(line * 2) * T::SZ * 3 + z * 3];
     float au[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) au[v] = s0[v] + s0[T::SZ * 3 + v];
-    const float4 uo = su[((tli + 1) * T::UBY + tlj + 1) * T::UBZ + z + 1];
+    const float4 uo = su[((tli + 1) * T::UBY + tj + 1) * T::UBZ + z + 1];
     const float own[3] = {uo.x, uo.y, uo.z};
     b.acc.b = ob[it];
     part += EpMode<MODE>::store(ep, b.acc, om[it], au, own, oin[it]);
@@ -531,9 +556,13 @@ __global__ void __launch_bounds__(64) k_mfma_probe(const uint4* __restrict__ a, 
 
 template <class T>
 int tile_launch_shape(const Grid& g, TileShape& ts, long long& nb) {
-  ts = TileShape{(g.NZ() + T::MTZ - 1) / T::MTZ, (g.NY() + T::MTY - 1) / T::MTY};
+  const int ntz = (g.NZ() + T::MTZ - 1) / T::MTZ, nty = (g.NY() + T::MTY - 1) / T::MTY;
+  auto magic = [](int d) { return d == 1 ? 0u : (uint32_t)(((1ull << 32) + (unsigned)d - 1) / (unsigned)d); };
+  ts = TileShape{ntz, nty, magic(ntz), magic(nty)};
   nb = (long long)((g.NX() + T::MTX - 1) / T::MTX) * ts.nty * ts.ntz;
   TG_REQUIRE(nb <= kMaxPartials, "grid too large for the reduction workspace");
+  // the tile index divisions (div_tiles) are exact while index * divisor < 2^32
+  TG_REQUIRE(nb * (long long)(ntz > nty ? ntz : nty) < (1LL << 32), "too many tiles");
   // every slab offset must stay below 2^31 bytes
   TG_REQUIRE((long long)(T::MTX + 2) * g.NY() * g.NZ() * 12 < (long long)kMaxRange,
              "grid cross-section %d x %d nodes too large for 32-bit slab offsets", g.NY(), g.NZ());

@@ -11,8 +11,9 @@ Restates the hierarchy of ``GalerkinMatFreeGMG`` (src/gpu_fem/multigrid_v4.py:
 * levels >= 2 are Galerkin element matrices built on the device from the
   level above (replacing the SpGEMM R @ (K @ P), multigrid_v4.py:747-752),
   assembled into 27-point block stencils for the V-cycle;
-* the coarsest level is solved with an FP64 dense Cholesky factorisation
-  turned into an explicit inverse once per setup (multigrid_v4.py:774-811).
+* the coarsest level is solved with an explicit FP64 dense inverse computed
+  once per setup (Gauss-Jordan elimination; it replaces the dense Cholesky
+  factorisation of multigrid_v4.py:774-811).
 
 Smoothing is Chebyshev-Jacobi on [lambda_max * lower_frac, lambda_max] with
 the coefficient recurrence of ``_chebyshev_steps`` (multigrid_v4.py:860-889),
