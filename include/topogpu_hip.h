@@ -34,7 +34,7 @@ enum {
   TG_ERR_ARG = 1,     /* invalid argument or unsupported shape            */
   TG_ERR_HIP = 2,     /* HIP runtime / kernel launch failure              */
   TG_ERR_STATE = 3,   /* tg_init not called / library state missing       */
-  TG_ERR_NOT_SPD = 4  /* dense SPD inverse met a non-positive pivot      */
+  TG_ERR_NOT_SPD = 4  /* dense SPD inverse: non-positive pivot          */
 };
 
 typedef struct {
@@ -197,8 +197,9 @@ int tg_assemble_dense(const tg_grid* g, const double* mats64, const int32_t* fre
                       int32_t n_free, double* A, void* stream);
 
 /* In-place explicit inverse A^-1 (full symmetric) of a dense FP64 SPD
- * matrix A (n x n, row-major), by Gauss-Jordan elimination without pivoting:
- * one workgroup for n <= 64, 16-wide pivot blocks with tiled updates above.
+ * matrix A (n x n, row-major): Gauss-Jordan elimination without pivoting in
+ * one workgroup (n <= 64) or in 16-wide pivot blocks with tiled updates
+ * (n <= 2048), rocSOLVER Cholesky factorisation and inverse above.
  * Host-synchronous. Replaces cp.linalg.cholesky + solve_triangular
  * (multigrid_v4.py:781-801, 977-978): the V-cycle then applies the coarsest
  * solve as one matrix-vector product. Returns TG_ERR_NOT_SPD when a pivot is

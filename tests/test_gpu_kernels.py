@@ -431,13 +431,17 @@ def test_not_spd_is_reported():
         B[bad, bad] = -1.0
         with pytest.raises(L.TopoHipError, match=f"not SPD \\(pivot {bad + 1}\\)"):
             L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), n, stream())
+    B = np.eye(2100)   # Cholesky path
+    B[1000, 1000] = -1.0
+    with pytest.raises(L.TopoHipError, match="not SPD"):
+        L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), 2100, stream())
 
 
-@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 135, 193, 600, 1080])
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 135, 193, 600, 1080, 2100])
 def test_dense_spd_inverse_sizes(n):
-    """One-workgroup Gauss-Jordan (n <= 64) and blocked Gauss-Jordan (64 x 64
-    blocks, n > 64) invert an ill-conditioned SPD matrix to FP64 accuracy,
-    symmetric."""
+    """One-workgroup Gauss-Jordan (n <= 64), blocked Gauss-Jordan (16-wide
+    pivot blocks, n <= 2048) and the library Cholesky inverse (above) invert
+    an ill-conditioned SPD matrix to FP64 accuracy, symmetric."""
     rng = np.random.default_rng(n)
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     ev = np.logspace(0, 8, n)   # condition number 1e8

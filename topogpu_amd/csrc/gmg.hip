@@ -8,6 +8,7 @@
 // solver_v2.py:62-149). P is the Kronecker product of the three axes, so the
 // normalisation factorises per axis.
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "common.h"
 #include "reduce.h"
@@ -627,6 +628,11 @@ extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const in
 }
 
 int dense_inverse_blocked(double* A, int n, hipStream_t st);
+int dense_inverse_cholesky(double* A, int n, hipStream_t st);
+// Above this size the 16-wide pivot steps of the blocked inverse (each a
+// pass over the whole matrix) cost more than rocSOLVER's Cholesky
+// factorisation and inverse (potrf + potri, 45 ms at n = 7,260 on MI355X).
+constexpr int kBlockedMaxN = 2048;
 
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
@@ -639,8 +645,10 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
     else
       hipLaunchKernelGGL((k_spd_inverse_gj<2>), dim3(1), dim3(GJ_THREADS), 0, st, A, n, n, A, n, info_d, 0, 1);
     TG_LAUNCH_CHECK();
-  } else {
+  } else if (n <= kBlockedMaxN) {
     return dense_inverse_blocked(A, n, st);
+  } else {
+    return dense_inverse_cholesky(A, n, st);
   }
   int info = 0;
   TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
@@ -676,6 +684,34 @@ int dense_inverse_blocked(double* A, int n, hipStream_t st) {
   TG_HIP_CHECK(hipStreamSynchronize(st));
   if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
   const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+// rocSOLVER Cholesky factorisation and inverse. A row-major symmetric A is
+// its own column-major transpose: work on the column-major lower triangle
+// (the row-major upper triangle), then mirror it.
+int dense_inverse_cholesky(double* A, int n, hipStream_t st) {
+  rocblas_handle h = nullptr;
+  if (int r = get_handle(st, &h)) return r;
+  rocblas_int* dinfo = reinterpret_cast<rocblas_int*>(workspace().flags);
+  rocblas_int info = 0;
+  rocblas_status rs = rocsolver_dpotrf(h, rocblas_fill_lower, n, A, n, dinfo);
+  if (rs == rocblas_status_success) {
+    TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+    TG_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if (rs == rocblas_status_success && info == 0) {
+    rs = rocsolver_dpotri(h, rocblas_fill_lower, n, A, n, dinfo);
+    if (rs == rocblas_status_success) {
+      TG_HIP_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+      TG_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  }
+  if (rs != rocblas_status_success) return set_error(TG_ERR_HIP, "rocsolver potrf/potri status %d", (int)rs);
+  if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (Cholesky info=%d)", (int)info);
+  const long long nn = (long long)n * n;
   hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
   TG_LAUNCH_CHECK();
   return TG_OK;

@@ -130,7 +130,8 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
   typename Ep::In ein;
   if (own) ep.load(n_own, ein);
   double part = 0.0;
-  if (own) {
+  if (own && sizeof(T) == 4) {
+    // FP32: all 27 neighbours in registers (81 values), each read from LDS once
     T nb[27][3];
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
@@ -166,6 +167,46 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
       }
     }
     const T own_u[3] = {nb[13][0], nb[13][1], nb[13][2]};
+    part = ep.store(n_own, m_own, acc, own_u, ein);
+  } else if (own) {
+    // FP64: one element at a time, its 8 corner displacements read from LDS
+    // per element (24 values in registers instead of 81 doubles), which
+    // leaves room for several waves per SIMD to hide the FMA and LDS
+    // latencies (2.7x faster than the 27-neighbour form at 160x80x80).
+    T acc[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int o = 0; o < 8; ++o) {
+      const int ox = corner_x(o), oy = corner_y(o), oz = corner_z(o);
+      const int eidx = ((tx - ox + 1) * EY + (ty - oy + 1)) * EZ + (tz - oz + 1);
+      const int ebase = ((tx - ox + 1) * BY + (ty - oy + 1)) * BZ + (tz - oz + 1);   // element's corner 0
+      T ue[8][3];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int idx = ebase + (corner_x(c) * BY + corner_y(c)) * BZ + corner_z(c);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ue[c][q] = su[3 * idx + q];
+      }
+#pragma unroll 1
+      for (int s = 0; s < NS; ++s) {
+        const T we = sw[eidx * NS + s];
+        const T* K = w.mat(s) + 72 * o;     // rows 3*o .. 3*o+2
+        T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const T uv = ue[c][q];
+            t0 = fma(K[0 * 24 + 3 * c + q], uv, t0);
+            t1 = fma(K[1 * 24 + 3 * c + q], uv, t1);
+            t2 = fma(K[2 * 24 + 3 * c + q], uv, t2);
+          }
+        acc[0] = fma(we, t0, acc[0]);
+        acc[1] = fma(we, t1, acc[1]);
+        acc[2] = fma(we, t2, acc[2]);
+      }
+    }
+    const int cidx = ((tx + 1) * BY + (ty + 1)) * BZ + tz + 1;
+    const T own_u[3] = {su[3 * cidx], su[3 * cidx + 1], su[3 * cidx + 2]};
     part = ep.store(n_own, m_own, acc, own_u, ein);
   }
   if (ep.reduces()) {

@@ -31,21 +31,26 @@ from .. import fem
 from . import lib as L
 from .gmg import GMG, launch_op
 
-# precision -> (Krylov operator, level-0 smoother operator,
-#               level-1 accurate operator, level-1 smoother operator)
+# precision -> (Krylov operator, level-0 smoother operator, level-0
+#               post-correction operator, level-1 post-correction operator,
+#               level-1 smoother operator)
 #   bf16       : split BF16 MFMA (hi/lo operands, 3 passes) for the PCG
-#                product and the post-correction products, single-pass BF16
-#                MFMA in the smoothers (levels 0 and 1)
+#                product, single-pass BF16 MFMA in the whole V-cycle (split
+#                post-corrections gave the same iteration counts on the
+#                160x80x80 workload at 2% more time per iteration)
 #   bf16-plain : single-pass BF16 MFMA everywhere (the inner solve then
 #                stalls near the BF16 rounding level; kept for study)
 #   bf16x3     : split BF16 MFMA everywhere
 #   fp32       : FP32 vector FMA everywhere (the plain FP32 path)
 # The level-1 entries apply when level 1 is the implicit Galerkin (Q8) form.
+# The post-correction product (r -= A c after the prolonged coarse
+# correction c) and the power iteration for lambda_max use the
+# post-correction operator of their level.
 PRECISIONS = {
-    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16),
-    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
-    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16X3),
-    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_Q8_F32, L.TG_OP_Q8_F32),
+    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
+    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
+    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16X3),
+    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_Q8_F32, L.TG_OP_Q8_F32),
 }
 
 
@@ -103,8 +108,8 @@ class HipStateSolver:
         self.inner_rtol, self.max_restarts = float(inner_rtol), int(max_restarts)
         self.warm_start = warm_start
         ops = PRECISIONS[precision]
-        self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[0],
-                       l1_q8_op=ops[3], l1_q8_accurate_op=ops[2], fine_degree=fine_degree,
+        self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[2],
+                       l1_q8_op=ops[4], l1_q8_accurate_op=ops[3], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
                        level1=level1, nrhs=self.nrhs, device=self.dev)
         nb = self.nrhs * self.n
