@@ -305,6 +305,25 @@ int tg_oc_finalize(int64_t n, const double* rho, const double* q, const double* 
                    const double* hi, const double* state, double* rho_new, double* change,
                    void* stream);
 
+/* Mean-volume OC of the reference GPU loop (src/topogpu/solver.py:94-117,
+ * `_oc_update`): no filter in the constraint, dc clamped to <= -1e-12, moves
+ * clipped to [max(rho - move, rho_min), min(rho + move, 1)], hi doubled from
+ * 1 while the mean stays above volfrac (hi < 1e40), then at most 80
+ * bisections with the early stop |mean - volfrac| < 1e-8.
+ * oc_mean_prepare: initialises state (TG_OC_MEAN_STATE_LEN doubles).
+ * oc_mean_step: one evaluation pass (64 multipliers: the next 64 doublings,
+ *   or the midpoints of the next 6 bisection levels) and the planner that
+ *   walks the loop through them; state[7] != 0 once the loop has stopped.
+ *   Calls after that return at once.
+ * oc_mean_finalize: rho_new = cand(last midpoint), *change = max|rho_new-rho|.
+ * The decisions and the last midpoint are those of the sequential loop. */
+#define TG_OC_MEAN_STATE_LEN 80
+int tg_oc_mean_prepare(double* state, void* stream);
+int tg_oc_mean_step(int64_t n, const double* rho, const double* dc, double volfrac, double move,
+                    double rho_min, double* state, void* stream);
+int tg_oc_mean_finalize(int64_t n, const double* rho, const double* dc, double move, double rho_min,
+                        const double* state, double* rho_new, double* change, void* stream);
+
 /* ---- batches of right-hand sides (several load cases at once) ----------
  * The multi-load case runs its load cases as one block through every solver
  * kernel: each call below is the single-RHS entry point of the same name

@@ -505,6 +505,41 @@ def test_density_chain_and_sensitivity():
     assert rel_inf(h(dcf), -3.0 * (1 - 1e-9) * rp ** 2 * ce_ref * dh) < 1e-13
 
 
+@pytest.mark.parametrize("scale,volfrac,move,spread", [
+    (1e-6, 0.5, 0.2, 3.0),      # multiplier below 1: bracket [0, 1], early stop
+    (1e6, 0.5, 0.2, 3.0),       # twenty doublings of hi
+    (1e25, 0.5, 0.2, 3.0),      # past 2^64: a second doubling pass
+    (1.0, 0.30, 0.01, 0.0),     # target out of reach: hi reaches 1e40, all 80 bisections
+])
+def test_oc_mean_matches_oracle(scale, volfrac, move, spread):
+    """Mean-volume OC of the reference GPU loop (solver.py:94-117) against
+    the oracle's restatement, bit for bit."""
+    rng = np.random.default_rng(17)
+    n = 12_345
+    rho = rng.uniform(0.05, 1.0, n) if spread else np.full(n, 0.5)
+    dc = -scale * 10.0 ** rng.uniform(-spread, 0.0, n)
+    dc[::97] = 0.0                  # clamped to -1e-12
+    ref = oracle.oc_update_mean(rho, dc, volfrac, move, 1e-3)
+    state = torch.zeros(L.OC_MEAN_STATE_LEN, dtype=torch.float64, device=DEV)
+    dr, ddc = t(rho), t(dc)
+    out = torch.zeros(n, dtype=torch.float64, device=DEV)
+    chg = torch.zeros(1, dtype=torch.float64, device=DEV)
+    L.call("tg_oc_mean_prepare", L.ptr(state), stream())
+    for _ in range(40):
+        L.call("tg_oc_mean_step", n, L.ptr(dr), L.ptr(ddc), volfrac, move, 1e-3, L.ptr(state), stream())
+    L.call("tg_oc_mean_finalize", n, L.ptr(dr), L.ptr(ddc), move, 1e-3, L.ptr(state), L.ptr(out), L.ptr(chg),
+           stream())
+    st = h(state)
+    assert st[7] == 1.0                                  # stopped
+    got = h(out)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    assert h(chg)[0] == np.abs(ref - rho).max()
+    if move == 0.01:
+        assert st[3] == 80                               # every bisection taken
+    else:
+        assert abs(ref.mean() - volfrac) < 1e-8
+
+
 @pytest.mark.parametrize("dims,seed,volfrac,spread", [
     ((10, 6, 5), 12, 0.45, 0.0),    # crossing inside the bracket
     ((10, 6, 5), 12, 0.30, 0.0),    # volume above target even at lambda = 1e9: all steps go right

@@ -547,6 +547,179 @@ __global__ void __launch_bounds__(256) k_max_finalize(int nb, double* out, Works
   if (threadIdx.x == 0) *out = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 }
 
+// ---------------------------------------------------------------------------
+// Mean-volume OC of the reference GPU loop (src/topogpu/solver.py:94-117):
+//   cand(lam) = clip(rho sqrt(max(-min(dc, -1e-12) / max(lam, 1e-40), 0)),
+//                    max(rho - move, rho_min), min(rho + move, 1));
+//   hi = 1, lo = 0; while mean(cand(hi)) > V and hi < 1e40: lo = hi, hi *= 2;
+//   80 times: mid = (lo + hi) / 2; mean(cand(mid)) > V ? lo = mid : hi = mid;
+//             stop when |mean(cand(mid)) - V| < 1e-8; result cand(mid).
+// Each pass evaluates the mean at up to OCM_MAXC multipliers at once: the
+// next 64 doublings of hi, or the 63 midpoints of the next six bisection
+// levels (every path the sequential loop can take, each midpoint computed by
+// the same floating-point operations). A one-block planner then walks the
+// sequential loop through the evaluated means, so the decisions and the
+// final multiplier are the loop's own. In the passes the candidate is
+// rho sqrt(-dc) / sqrt(lam) (one rounding apart from the loop's form, which
+// moves a mean by ~1e-16 of itself); the final density uses the loop's
+// exact expression.
+constexpr int OCM_MAXC = 64, OCM_DEPTH = 6;
+enum OcmState {
+  M_PHASE,   // 0: doubling hi, 1: bisection, 2: done
+  M_LO, M_HI,
+  M_STEPS,   // bisection steps taken
+  M_LAM,     // final multiplier
+  M_KBASE,   // first doubling exponent of the pass
+  M_NC,      // candidates in the pass
+  M_DONE,
+  M_CAND,
+  M_LEN = M_CAND + OCM_MAXC
+};
+static_assert(M_LEN <= TG_OC_MEAN_STATE_LEN, "mean-OC state layout");
+
+__device__ __forceinline__ double ocm_exact(double rho, double dc, double move, double rho_min, double lam) {
+  const double dcs = fmin(dc, -1.0e-12);
+  const double x = rho * sqrt(fmax(-dcs / fmax(lam, 1.0e-40), 0.0));
+  return fmin(fmax(x, fmax(rho - move, rho_min)), fmin(rho + move, 1.0));
+}
+
+// the 63 midpoints of the bisection subtree below (lo, hi), heap order:
+// node p has children 2p+1 (hi = mid) and 2p+2 (lo = mid)
+__device__ void ocm_tree(double lo, double hi, double* cand) {
+  double L[OCM_MAXC], H[OCM_MAXC];
+  L[0] = lo;
+  H[0] = hi;
+  constexpr int nodes = (1 << OCM_DEPTH) - 1;
+  for (int p = 0; p < nodes; ++p) {
+    const double mid = 0.5 * (L[p] + H[p]);
+    cand[p] = mid;
+    if (2 * p + 2 < nodes) {
+      L[2 * p + 1] = L[p];
+      H[2 * p + 1] = mid;
+      L[2 * p + 2] = mid;
+      H[2 * p + 2] = H[p];
+    }
+  }
+}
+
+__global__ void k_ocm_prepare(double* state) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < M_LEN; ++k) state[k] = 0.0;
+  state[M_HI] = 1.0;
+  state[M_NC] = OCM_MAXC;
+  for (int k = 0; k < OCM_MAXC; ++k) state[M_CAND + k] = ldexp(1.0, k);
+}
+
+// Per block and candidate: the sum of the candidate densities of the
+// block's elements (grid-stride, fixed assignment), 16 candidates at a time.
+__global__ void __launch_bounds__(256) k_ocm_pass(long long n, const double* __restrict__ rho,
+                                                  const double* __restrict__ dc, double move, double rho_min,
+                                                  const double* state, Workspace ws) {
+  if (state[M_DONE] != 0.0) return;   // uniform
+  const int nc = (int)state[M_NC], nb = gridDim.x;
+  for (int c0 = 0; c0 < nc; c0 += 16) {
+    double rs[16], acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      rs[c] = c0 + c < nc ? 1.0 / sqrt(fmax(state[M_CAND + c0 + c], 1.0e-40)) : 0.0;
+      acc[c] = 0.0;
+    }
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)nb * blockDim.x) {
+      const double r = rho[e], q = r * sqrt(-fmin(dc[e], -1.0e-12));
+      const double lo = fmax(r - move, rho_min), hi = fmin(r + move, 1.0);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc[c] += fmin(fmax(q * rs[c], lo), hi);
+    }
+    block_sum<16>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (c0 + c < nc) ws.partials[(size_t)(c0 + c) * nb + blockIdx.x] = acc[c];
+  }
+}
+
+// One block: candidate means (partials summed in block order), then the
+// sequential loop walked through them; writes the next candidates.
+__global__ void __launch_bounds__(256) k_ocm_plan(int nb, long long n, double volfrac, double* state, Workspace ws) {
+  if (state[M_DONE] != 0.0) return;
+  __shared__ double mean[OCM_MAXC];
+  const int nc = (int)state[M_NC];
+  for (int c = 0; c < nc; ++c) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ws.partials[(size_t)c * nb + b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) mean[c] = v[0] / (double)n;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (state[M_PHASE] == 0.0) {
+    // candidates are hi = 2^(kbase + c): the loop moves on while the mean
+    // stays above the target and hi < 1e40
+    const int kb = (int)state[M_KBASE];
+    for (int c = 0; c < nc; ++c) {
+      const double hi = state[M_CAND + c];
+      if (mean[c] > volfrac && hi < 1.0e40) {
+        state[M_LO] = hi;
+        continue;
+      }
+      state[M_HI] = hi;
+      state[M_PHASE] = 1.0;
+      ocm_tree(state[M_LO], hi, state + M_CAND);
+      state[M_NC] = (1 << OCM_DEPTH) - 1;
+      return;
+    }
+    state[M_KBASE] = kb + nc;
+    for (int c = 0; c < OCM_MAXC; ++c) state[M_CAND + c] = ldexp(1.0, kb + nc + c);
+    state[M_NC] = OCM_MAXC;
+    return;
+  }
+  // bisection: walk the subtree
+  double lo = state[M_LO], hi = state[M_HI];
+  int steps = (int)state[M_STEPS];
+  int p = 0;
+  for (int d = 0; d < OCM_DEPTH; ++d) {
+    const double mid = state[M_CAND + p];
+    const double m = mean[p];
+    ++steps;
+    if (m > volfrac) {
+      lo = mid;
+      p = 2 * p + 2;
+    } else {
+      hi = mid;
+      p = 2 * p + 1;
+    }
+    if (fabs(m - volfrac) < 1.0e-8 || steps >= 80) {
+      state[M_LAM] = mid;
+      state[M_STEPS] = steps;
+      state[M_DONE] = 1.0;
+      state[M_PHASE] = 2.0;
+      return;
+    }
+  }
+  state[M_LO] = lo;
+  state[M_HI] = hi;
+  state[M_STEPS] = steps;
+  ocm_tree(lo, hi, state + M_CAND);
+}
+
+__global__ void __launch_bounds__(256) k_ocm_finalize(long long n, const double* __restrict__ rho,
+                                                      const double* __restrict__ dc, double move, double rho_min,
+                                                      const double* state, double* rho_new, Workspace ws) {
+  const double lam = state[M_LAM];
+  double mx = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const double r = ocm_exact(rho[e], dc[e], move, rho_min, lam);
+    rho_new[e] = r;
+    mx = fmax(mx, fabs(r - rho[e]));
+  }
+  __shared__ double smax[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) ws.partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+}
+
 int grid_blocks(long long n) {
   long long b = (n + 255) / 256;
   if (b > 2048) b = 2048;
@@ -650,6 +823,42 @@ extern "C" int tg_oc_bisect_step(const tg_grid* gp, int32_t m, const int32_t* of
   TG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(256), 0, (hipStream_t)s, nb, g.elems(), volfrac, 100, state,
                      workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_oc_mean_prepare(double* state, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(state, "oc_mean_prepare: NULL state");
+  hipLaunchKernelGGL(k_ocm_prepare, dim3(1), dim3(64), 0, (hipStream_t)s, state);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_oc_mean_step(int64_t n, const double* rho, const double* dc, double volfrac, double move,
+                               double rho_min, double* state, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && rho && dc && state, "oc_mean_step: NULL argument");
+  const int nb = grid_blocks(n) < 1024 ? grid_blocks(n) : 1024;
+  TG_REQUIRE((long long)nb * OCM_MAXC <= kMaxPartials, "oc_mean_step: workspace too small");
+  hipLaunchKernelGGL(k_ocm_pass, dim3(nb), dim3(256), 0, (hipStream_t)s, (long long)n, rho, dc, move, rho_min, state,
+                     workspace());
+  TG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_ocm_plan, dim3(1), dim3(256), 0, (hipStream_t)s, nb, (long long)n, volfrac, state,
+                     workspace());
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_oc_mean_finalize(int64_t n, const double* rho, const double* dc, double move, double rho_min,
+                                   const double* state, double* rho_new, double* change, void* s) {
+  TG_NEED_INIT();
+  TG_REQUIRE(n > 0 && rho && dc && state && rho_new && change, "oc_mean_finalize: NULL argument");
+  const int nb = grid_blocks(n);
+  hipLaunchKernelGGL(k_ocm_finalize, dim3(nb), dim3(256), 0, (hipStream_t)s, (long long)n, rho, dc, move, rho_min,
+                     state, rho_new, workspace());
+  TG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_finalize, dim3(1), dim3(256), 0, (hipStream_t)s, nb, change, workspace());
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

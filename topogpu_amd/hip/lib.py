@@ -17,6 +17,7 @@ TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32,
 TG_OP_Q8_BF16, TG_OP_Q8_BF16X3 = 7, 8
 TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE = range(7)
 OC_STATE_LEN = 32   # device doubles of the OC bisection state (TG_OC_STATE_LEN)
+OC_MEAN_STATE_LEN = 80   # ... of the mean-volume OC (TG_OC_MEAN_STATE_LEN)
 
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -73,6 +74,9 @@ _SIGS = {
     "tg_fill64": [_i64, _vp, _f64, _vp],
     "tg_copy32": [_i64, _vp, _vp, _vp],
     "tg_compliance_dd": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp],
+    "tg_oc_mean_prepare": [_vp, _vp],
+    "tg_oc_mean_step": [_i64, _vp, _vp, _f64, _f64, _f64, _vp, _vp],
+    "tg_oc_mean_finalize": [_i64, _vp, _vp, _f64, _f64, _vp, _vp, _vp, _vp],
     "tg_apply32_b": [C.POINTER(TgOp32), _i32, _vp, _vp, C.POINTER(TgEp32), _vp],
     "tg_apply64_b": [C.POINTER(TgGrid), _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_restrict32_b": [C.POINTER(TgGrid), C.POINTER(TgGrid), _vp, _i32, _vp, _vp, _vp],

@@ -1,9 +1,15 @@
 """One SIMP design step on the device: filter -> projection -> modulus ->
 state solve -> compliance sensitivity -> filter transpose -> OC bisection.
 
-Follows ``_fea_and_sensitivity`` and ``_oc_update`` of the reference CPU path
-(src/gpu_fem/pub_simp_solver.py:319-377). Everything except the state solve's
-convergence test and a few scalars stays on the GPU.
+``formulation="filtered"`` (default) follows ``_fea_and_sensitivity`` and
+``_oc_update`` of the reference CPU path (src/gpu_fem/pub_simp_solver.py:
+319-377): density filter, Heaviside projection, OC on the filtered volume.
+``formulation="solverv4"`` follows the reference GPU loop
+(src/topogpu/solver.py:94-117, 186-249): the design density is the physical
+density (no filter, no projection), dc is SolverV4's element sensitivity
+(solver_v4.py:246-251) and the OC bisects on the plain mean volume with the
+``rho_min`` bound. Everything except the state solve's convergence test and a
+few scalars stays on the GPU.
 """
 
 from __future__ import annotations
@@ -34,7 +40,11 @@ class StepResult:
 class HipSIMP:
     def __init__(self, dims, free, F, *, volfrac: float, rmin: float = 1.5, penal: float = 3.0,
                  move: float = 0.2, use_heaviside: bool = True, beta: float = 1.0, eta: float = 0.5,
-                 rho0: np.ndarray | None = None, weights=None, **solver_kw):
+                 rho0: np.ndarray | None = None, weights=None, formulation: str = "filtered",
+                 rho_min: float = 1.0e-3, **solver_kw):
+        if formulation not in ("filtered", "solverv4"):
+            raise ValueError(f"formulation must be 'filtered' or 'solverv4'; got {formulation!r}")
+        self.formulation, self.rho_min = formulation, float(rho_min)
         self.dims = tuple(int(d) for d in dims)
         self.n_elem = int(np.prod(self.dims))
         self.volfrac, self.rmin, self.penal, self.move = float(volfrac), float(rmin), float(penal), float(move)
@@ -69,7 +79,7 @@ class HipSIMP:
         self.q = torch.zeros(self.n_elem, **f64)
         self.lo = torch.zeros(self.n_elem, **f64)
         self.hi = torch.zeros(self.n_elem, **f64)
-        self.oc_state = torch.zeros(L.OC_STATE_LEN, **f64)
+        self.oc_state = torch.zeros(max(L.OC_STATE_LEN, L.OC_MEAN_STATE_LEN), **f64)
         self.scal = torch.zeros(4, **f64)
 
     def _row_sums(self) -> np.ndarray:
@@ -98,8 +108,15 @@ class HipSIMP:
                L.ptr(ins), L.ptr(outs), L.ptr(out), self.stream)
 
     def physical(self, rho=None) -> None:
-        """rho_f = H rho; rho_phys, dh, E (pub_simp_solver.py:347-350)."""
-        self.filter(self.rho if rho is None else rho, self.rho_f)
+        """rho_f = H rho; rho_phys, dh, E (pub_simp_solver.py:347-350); with
+        the solverv4 formulation rho_phys = rho and E = Emin + (E0 - Emin)
+        rho^p (solver_v4.py:167-168)."""
+        rho = self.rho if rho is None else rho
+        if self.formulation == "solverv4":
+            L.call("tg_density_chain", self.n_elem, L.ptr(rho), 0.0, self.eta, self.penal, EMIN, E0,
+                   L.ptr(self.rho_phys), L.ptr(self.dh), L.ptr(self.E64), L.ptr(self.E32), self.stream)
+            return
+        self.filter(rho, self.rho_f)
         beta = self.beta if self.use_heaviside else 0.0
         L.call("tg_density_chain", self.n_elem, L.ptr(self.rho_f), beta, self.eta, self.penal, EMIN, E0,
                L.ptr(self.rho_phys), L.ptr(self.dh), L.ptr(self.E64), L.ptr(self.E32), self.stream)
@@ -114,13 +131,42 @@ class HipSIMP:
     def sensitivities(self) -> None:
         """dc = H^T(dc_phys * dh), dv = H^T dh (pub_simp_solver.py:361-371);
         with several load cases dc_phys is the weighted sum over the cases."""
+        if self.formulation == "solverv4":
+            # dc = -p (E0 - Emin) rho^(p-1) u_e^T KE u_e (solver_v4.py:246-251)
+            L.call("tg_sensitivity64_b", self.grid, self.fea.nrhs, L.ptr(self.fea.u), L.ptr(self.weights),
+                   L.ptr(self.rho_phys), None, self.penal, E0 - EMIN, None, L.ptr(self.dc), self.stream)
+            return
         L.call("tg_sensitivity64_b", self.grid, self.fea.nrhs, L.ptr(self.fea.u), L.ptr(self.weights),
                L.ptr(self.rho_phys), L.ptr(self.dh), self.penal, E0 - EMIN, None, L.ptr(self.dcf), self.stream)
         self.filter(self.dcf, self.dc, transpose=True)
         self.filter(self.dh, self.dv, transpose=True)
 
+    def oc_mean(self) -> tuple[float, int]:
+        """Mean-volume OC of the reference GPU loop (solver.py:94-117)."""
+        st = self.stream
+        L.call("tg_oc_mean_prepare", L.ptr(self.oc_state), st)
+        # one doubling pass, then six bisection steps per pass (80 at most);
+        # calls after the stop return at once, so a few are queued per check
+        passes = 0
+        while passes < 40:
+            for _ in range(4):
+                L.call("tg_oc_mean_step", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.volfrac, self.move,
+                       self.rho_min, L.ptr(self.oc_state), st)
+            passes += 4
+            if float(self.oc_state[7].item()):
+                break
+        L.call("tg_oc_mean_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.move, self.rho_min,
+               L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
+        change = float(self.scal[0].item())
+        n_steps = int(self.oc_state[3].item())
+        self.rho, self.rho_new = self.rho_new, self.rho
+        return change, n_steps
+
     def oc(self, max_steps: int = 100) -> tuple[float, int]:
-        """Filtered-volume OC bisection (pub_simp_solver.py:319-340)."""
+        """Filtered-volume OC bisection (pub_simp_solver.py:319-340); the
+        mean-volume update with the solverv4 formulation."""
+        if self.formulation == "solverv4":
+            return self.oc_mean()
         st = self.stream
         L.call("tg_oc_prepare", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), L.ptr(self.dv), self.move,
                L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), L.ptr(self.oc_state), st)

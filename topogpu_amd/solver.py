@@ -7,8 +7,12 @@ on an MI355X: the reference's CPU algorithm (``run_simp``, density filter +
 Heaviside projection + filtered-volume OC, src/gpu_fem/pub_simp_solver.py:
 412-657) with the state solve replaced by the GMG-PCG of ``topogpu_amd.hip``.
 BCs and loads come from the problem itself (the reference's ``run_simp``
-``bc_override`` path). Other backend names are not served here: the
-reference keeps serving them.
+``bc_override`` path). With ``formulation="solverv4"`` it runs the loop of
+the reference GPU backend instead (``_solve_solverv4``, src/topogpu/
+solver.py:186-249): no filter or projection, mean-volume OC with
+``rho_min``, exactly ``max_iter`` iterations, the design density as
+``rho_final``. Other backend names are not served here: the reference keeps
+serving them.
 """
 
 from __future__ import annotations
@@ -67,7 +71,9 @@ class SIMPSolver:
 
     Extra keyword options of the hip backend (all optional):
     precision ("bf16" | "bf16x3" | "fp32"), n_levels, fine_degree,
-    coarse_degree, inner_rtol, use_heaviside, beta, min_iter, conv_tol.
+    coarse_degree, inner_rtol, use_heaviside, beta, min_iter, conv_tol,
+    formulation ("filtered": the reference CPU algorithm, default;
+    "solverv4": the reference GPU loop), weights (multi-load).
     """
 
     def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
@@ -99,6 +105,8 @@ class SIMPSolver:
         if not problem.is_3d:
             raise ValueError("The hip backend currently requires a 3D problem.")
         opts = dict(self.hip_options)
+        if opts.get("formulation", "filtered") == "solverv4":
+            return self._solve_hip_solverv4(problem, opts)
         min_iter = int(opts.pop("min_iter", min(2, self.max_iter)))
         conv_tol = float(opts.pop("conv_tol", 1.0e-3))
         compliance_tol = float(opts.pop("compliance_tol", 5.0e-4))
@@ -136,6 +144,42 @@ class SIMPSolver:
             "precision": engine.fea.precision, "n_levels": len(engine.fea.gmg.levels), "n_iter": iteration,
             "total_wall_s": time.perf_counter() - t_total, "final_compliance": float(final_c),
             "final_grayness": grayness(rho_final), "final": history[-1] if history else {},
+        }
+        return OptimizationResult(problem, rho_final, history, summary)
+
+
+    def _solve_hip_solverv4(self, problem: TopologyProblem, opts: dict[str, Any]) -> OptimizationResult:
+        """The reference GPU loop (solver.py:186-249) on the device: solve,
+        mean-volume OC, one history row per iteration, max_iter iterations."""
+        from .hip.simp import HipSIMP
+
+        spec = problem.spec
+        bc = generate_bc(spec)
+        force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
+        rmin = problem.filter_radius or spec.rmin or 1.5   # unused by the formulation; kept for the engine
+        engine = HipSIMP(problem.nel, bc.free_dofs, force, volfrac=spec.volfrac, rmin=rmin, move=self.move,
+                         rho_min=self.rho_min, tol=self.tol, max_krylov=self.max_krylov, **opts)
+        history: list[dict[str, Any]] = []
+        t_total = time.perf_counter()
+        for iteration in range(1, self.max_iter + 1):
+            t0 = time.perf_counter()
+            step = engine.step()
+            wall = time.perf_counter() - t0
+            rho = engine.rho.cpu().numpy()
+            history.append({
+                "iteration": iteration, "compliance": float(step.compliance), "outer_iters": int(step.iters),
+                "rho_mean": float(rho.mean()), "rho_min": float(rho.min()), "rho_max": float(rho.max()),
+                "grayness": grayness(rho), "wall_s": wall, "outer_solver": "pcg",
+                "change": float(step.change), "ir_restarts": int(step.restarts),
+                "rel_residual": float(step.rel_residual), "converged": bool(step.converged),
+            })
+        rho_final = engine.rho.cpu().numpy()
+        summary = {
+            "problem": problem.to_dict(), "backend": "hip",
+            "linear_solver": "pcg_gmg" if self.linear_solver == "auto" else self.linear_solver,
+            "formulation": "solverv4", "precision": engine.fea.precision, "n_levels": len(engine.fea.gmg.levels),
+            "n_iter": self.max_iter, "total_wall_s": time.perf_counter() - t_total,
+            "final": history[-1] if history else {},
         }
         return OptimizationResult(problem, rho_final, history, summary)
 
