@@ -460,8 +460,8 @@ def test_dense_spd_inverse_sizes(n):
 
 
 @pytest.mark.parametrize("rmin", [1.5, 2.0, 2.7])
-def test_filter_and_transpose(rmin):
-    dims = (9, 5, 7)
+@pytest.mark.parametrize("dims", [(9, 5, 7), (20, 17, 35)])
+def test_filter_and_transpose(rmin, dims):
     n = int(np.prod(dims))
     H = oracle.build_density_filter(*dims, rmin)
     offs, w = fem.filter_stencil(rmin)

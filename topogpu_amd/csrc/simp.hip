@@ -55,9 +55,27 @@ __global__ void k_stencil(Grid g, Stencil st, const double* __restrict__ in, con
   out[e] = out_scale ? s * out_scale[e] : s;
 }
 
+// FP64 tanh within about one ulp: for |y| < 0.625 the rational
+// approximation y + y^3 P(y^2)/Q(y^2) of S. L. Moshier's Cephes library
+// (tanh.c), else 1 - 2/(exp(2|y|) + 1) with the sign of y. The Heaviside
+// projection evaluates it once per element and trial multiplier in the OC
+// passes (beta = 1 keeps |y| <= 1/2, the rational branch); the generic
+// library routine there cost most of the pass.
+__device__ __forceinline__ double tanh_d(double y) {
+  const double a = fabs(y);
+  if (a < 0.625) {
+    const double z = y * y;
+    const double p = (-9.64399179425052238628e-1 * z - 9.92877231001918586564e1) * z - 1.61468768441708447952e3;
+    const double q = ((z + 1.12811678491632931402e2) * z + 2.23548839060100448583e3) * z + 4.84406305325125486048e3;
+    return y + y * z * p / q;
+  }
+  const double t = exp(-2.0 * a);
+  return copysign((1.0 - t) / (1.0 + t), y);
+}
+
 __device__ __forceinline__ double proj(double x, double beta, double eta) {
-  const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
-  return (tanh(beta * eta) + tanh(beta * (x - eta))) / den;
+  const double den = tanh_d(beta * eta) + tanh_d(beta * (1.0 - eta));
+  return (tanh_d(beta * eta) + tanh_d(beta * (x - eta))) / den;
 }
 
 __global__ void k_density_chain(long long n, const double* __restrict__ rf, double beta, double eta, double penal,
@@ -67,9 +85,9 @@ __global__ void k_density_chain(long long n, const double* __restrict__ rf, doub
   const double x = rf[e];
   double p = x, d = 1.0;
   if (beta >= 1e-6) {
-    const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
-    const double th = tanh(beta * (x - eta));
-    p = (tanh(beta * eta) + th) / den;
+    const double den = tanh_d(beta * eta) + tanh_d(beta * (1.0 - eta));
+    const double th = tanh_d(beta * (x - eta));
+    p = (tanh_d(beta * eta) + th) / den;
     d = beta * (1.0 - th * th) / den;
   }
   const double E = emin + (e0 - emin) * pow(p, penal);
@@ -331,7 +349,7 @@ __global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st
   const int k0 = (bt % ntz) * OTZ, j0 = ((bt / ntz) % nty) * OTY, i0 = (bt / (ntz * nty)) * OTX;
 
   const bool proj_on = use_h && beta > 1e-6;
-  const double p0 = tanh(beta * eta), pden = p0 + tanh(beta * (1.0 - eta));
+  const double p0 = tanh_d(beta * eta), ipden = 1.0 / (p0 + tanh_d(beta * (1.0 - eta)));
   // this thread's z-run
   const int tz = threadIdx.x % (OTZ / OC_RUN), ty = (threadIdx.x / (OTZ / OC_RUN)) % OTY,
             tx = threadIdx.x / ((OTZ / OC_RUN) * OTY);
@@ -346,9 +364,10 @@ __global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st
   const int obase = ((tx + R) * HY + (ty + R)) * HZ + tz * OC_RUN + R;
 
   const int nb = gridDim.x;
-#pragma unroll 1
-  for (int grp = 0; grp < OC_NPTS / OC_G; ++grp) {
-    if (grp > 0) __syncthreads();   // previous group's reads are done
+  // one group of OC_G multipliers per block row: the tiles' groups run as
+  // independent workgroups, which spreads the pass over many more of them
+  {
+    const int grp = blockIdx.y;
     double rs[OC_G];
 #pragma unroll
     for (int c = 0; c < OC_G; ++c) rs[c] = 1.0 / sqrt(state[S_CAND + grp * OC_G + c]);
@@ -388,7 +407,7 @@ __global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st
       for (int r = 0; r < OC_RUN; ++r)
         if (own[r]) {
           const double rf = acc[c][r] * hinv[r];
-          vg[c] += proj_on ? (p0 + tanh(beta * (rf - eta))) / pden : rf;
+          vg[c] += proj_on ? (p0 + tanh_d(beta * (rf - eta))) * ipden : rf;
         }
     }
     // partials in the layout of block_partials<OC_NPTS>
@@ -419,16 +438,76 @@ __global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st
 //     the crossing so that whole runs of later midpoints are decided. Once
 //     [a, b] is narrower than the bisection's final resolution, the pass
 //     evaluates the 15 midpoints of the next four levels instead.
+// The filter on OTX x OTY x OTZ element tiles: the (scaled) input of the
+// tile and its R-element halo is staged in LDS once, then each thread sums a
+// z-run of OC_RUN elements. Terms are added in the stencil's order and an
+// out-of-grid neighbour adds w * 0, so the result is that of k_stencil.
+__global__ void __launch_bounds__(OC_THREADS) k_stencil_tiled(Grid g, Stencil st, int R, const double* __restrict__ in,
+                                                             const double* __restrict__ in_scale,
+                                                             const double* __restrict__ out_scale,
+                                                             double* __restrict__ out) {
+  constexpr int HMAX = (OTX + 2 * OC_R) * (OTY + 2 * OC_R) * (OTZ + 2 * OC_R);
+  constexpr int HIT = (HMAX + OC_THREADS - 1) / OC_THREADS;
+  __shared__ double sx[HMAX];
+  const int HX = OTX + 2 * R, HY = OTY + 2 * R, HZ = OTZ + 2 * R, NH = HX * HY * HZ;
+  const int ntz = (g.nz + OTZ - 1) / OTZ, nty = (g.ny + OTY - 1) / OTY;
+  const int bt = blockIdx.x;
+  const int k0 = (bt % ntz) * OTZ, j0 = ((bt / ntz) % nty) * OTY, i0 = (bt / (ntz * nty)) * OTX;
+#pragma unroll
+  for (int it = 0; it < HIT; ++it) {
+    const int idx = threadIdx.x + it * OC_THREADS;
+    const int hz = idx % HZ, hy = (idx / HZ) % HY, hx = idx / (HZ * HY);
+    const int i = i0 - R + hx, j = j0 - R + hy, k = k0 - R + hz;
+    const bool in_g = idx < NH && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+    const long long f = in_g ? g.elem(i, j, k) : 0;
+    const double v = in_g ? (in_scale ? in[f] * in_scale[f] : in[f]) : 0.0;
+    if (idx < NH) sx[idx] = v;
+  }
+  __syncthreads();
+  const int tz = threadIdx.x % (OTZ / OC_RUN), ty = (threadIdx.x / (OTZ / OC_RUN)) % OTY,
+            tx = threadIdx.x / ((OTZ / OC_RUN) * OTY);
+  const int oi = i0 + tx, oj = j0 + ty, ok0 = k0 + tz * OC_RUN;
+  const int obase = ((tx + R) * HY + (ty + R)) * HZ + tz * OC_RUN + R;
+  double acc[OC_RUN];
+#pragma unroll
+  for (int r = 0; r < OC_RUN; ++r) acc[r] = 0.0;
+  for (int o = 0; o < st.m; ++o) {
+    const int off = obase + (st.off[o][0] * HY + st.off[o][1]) * HZ + st.off[o][2];
+    const double w = st.w[o];
+#pragma unroll
+    for (int r = 0; r < OC_RUN; ++r) acc[r] = fma(w, sx[off + r], acc[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < OC_RUN; ++r) {
+    if (oi < g.nx && oj < g.ny && ok0 + r < g.nz) {
+      const long long e = g.elem(oi, oj, ok0 + r);
+      out[e] = out_scale ? acc[r] * out_scale[e] : acc[r];
+    }
+  }
+}
+
+// out[c] = sum over blocks b of partials[c * nb + b] for c < nc, 16 lanes of
+// a 256-thread block per candidate (16 candidates per round), each lane
+// summing a fixed stride of blocks, then the lanes in a fixed tree.
+__device__ void sum_candidate_partials(const double* partials, int nb, int nc, double* out) {
+  const int c16 = threadIdx.x >> 4, l = threadIdx.x & 15;
+  for (int c0 = 0; c0 < nc; c0 += 16) {
+    const int c = c0 + c16;
+    double v = 0.0;
+    if (c < nc)
+      for (int b = l; b < nb; b += 16) v += partials[(size_t)c * nb + b];
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+    if (c < nc && l == 0) out[c] = v;
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double volfrac, int max_steps, double* state,
                                                 Workspace ws) {
   if (state[S_DONE] != 0.0) return;
   __shared__ double svol[OC_NPTS];
-  for (int t = 0; t < OC_NPTS; ++t) {
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ws.partials[(size_t)t * nb + b];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) svol[t] = v[0];
-  }
+  sum_candidate_partials(ws.partials, nb, OC_NPTS, svol);
   if (threadIdx.x != 0) return;
   const double dn = (double)n;
   constexpr double kMargin = 2e-14;   // on volume/n; a faithfully rounded projection moves it by ~1e-16
@@ -644,14 +723,9 @@ __global__ void __launch_bounds__(256) k_ocm_plan(int nb, long long n, double vo
   if (state[M_DONE] != 0.0) return;
   __shared__ double mean[OCM_MAXC];
   const int nc = (int)state[M_NC];
-  for (int c = 0; c < nc; ++c) {
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ws.partials[(size_t)c * nb + b];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) mean[c] = v[0] / (double)n;
-  }
-  __syncthreads();
+  sum_candidate_partials(ws.partials, nb, nc, mean);
   if (threadIdx.x != 0) return;
+  for (int c = 0; c < nc; ++c) mean[c] /= (double)n;
   if (state[M_PHASE] == 0.0) {
     // candidates are hi = 2^(kbase + c): the loop moves on while the mean
     // stays above the target and hi < 1e40
@@ -741,8 +815,19 @@ extern "C" int tg_stencil64(const tg_grid* gp, int32_t m, const int32_t* offs, c
   Stencil st;
   if (int r = make_stencil(m, offs, w, st)) return r;
   const Grid g = to_grid(gp);
-  hipLaunchKernelGGL(k_stencil, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, st, in,
-                     in_scale, out_scale, out);
+  int R = 0;
+  for (int k = 0; k < st.m; ++k)
+    for (int d = 0; d < 3; ++d) R = abs(st.off[k][d]) > R ? abs(st.off[k][d]) : R;
+  if (R <= OC_R) {
+    const long long tiles =
+        (long long)((g.nx + OTX - 1) / OTX) * ((g.ny + OTY - 1) / OTY) * ((g.nz + OTZ - 1) / OTZ);
+    TG_REQUIRE(tiles < (1LL << 31), "stencil64: grid too large");
+    hipLaunchKernelGGL(k_stencil_tiled, dim3((unsigned)tiles), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R, in,
+                       in_scale, out_scale, out);
+  } else {
+    hipLaunchKernelGGL(k_stencil, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, st,
+                       in, in_scale, out_scale, out);
+  }
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
@@ -813,7 +898,8 @@ extern "C" int tg_oc_bisect_step(const tg_grid* gp, int32_t m, const int32_t* of
     const long long tiles = (long long)((g.nx + OTX - 1) / OTX) * ((g.ny + OTY - 1) / OTY) * ((g.nz + OTZ - 1) / OTZ);
     TG_REQUIRE(tiles * OC_NPTS <= kMaxPartials, "grid too large for the reduction workspace");
     nb = (int)tiles;
-    hipLaunchKernelGGL(k_oc_pass_tiled, dim3(nb), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R, hs_inv, q, lo, hi,
+    hipLaunchKernelGGL(k_oc_pass_tiled, dim3(nb, OC_NPTS / OC_G), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R,
+                       hs_inv, q, lo, hi,
                        beta, eta, use_h, state, workspace());
   } else {
     nb = grid_blocks(g.elems());
