@@ -49,7 +49,8 @@ C1 = (128, 64, 64)
 C4 = dict(dims=(192, 64, 64), n_loads=8, seed=3, volfrac=0.30, rmin=1.5, move=0.2, n_levels="auto", fine_degree=2,
           coarse_degree=(1, 4), tol=1e-5, max_krylov=800)
 HBM_PEAK_GBPS = 8000.0
-EP_NAMES = {0: "STORE", 1: "DOT", 2: "RESID", 3: "RESID_NORM", 4: "CHEB", 5: "CHEB_R", 6: "RESID_UPDATE"}
+EP_NAMES = {0: "STORE", 1: "DOT", 2: "RESID", 3: "RESID_NORM", 4: "CHEB", 5: "CHEB_R", 6: "RESID_UPDATE",
+            7: "CHEB_FIRST", 8: "RESID_FIRST", 9: "CHEB_LAST"}
 
 
 def _log(msg: str) -> None:
@@ -173,6 +174,10 @@ def saturated_kernel_times(eng, counter, reps: int = 50) -> dict:
     eps = {
         L.TG_EP_CHEB_R: (L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1), None),
         L.TG_EP_RESID_UPDATE: (L.ep(L.TG_EP_RESID_UPDATE, r=r), None),
+        L.TG_EP_CHEB_FIRST: (L.ep(L.TG_EP_CHEB_FIRST, rhs=u, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1, a3=0.5),
+                             None),
+        L.TG_EP_RESID_FIRST: (L.ep(L.TG_EP_RESID_FIRST, rhs=u, dinv=lv0.dinv, x=x, r=r, a3=0.5), None),
+        L.TG_EP_CHEB_LAST: (L.ep(L.TG_EP_CHEB_LAST, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1), None),
         L.TG_EP_DOT: (L.ep(L.TG_EP_DOT, dot=dot), y),
         L.TG_EP_STORE: (L.ep(L.TG_EP_STORE), y),
     }

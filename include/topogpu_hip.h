@@ -96,7 +96,19 @@ enum {
   TG_EP_CHEB_R = 5,     /* residual-update Chebyshev step on a correction u:
                            r -= A u; d = a1*dinv*r + a2*u; x += a3*u + d
                            (r, x in/out; d output, d != u)                  */
-  TG_EP_RESID_UPDATE = 6 /* r -= A u (r in/out)                             */
+  TG_EP_RESID_UPDATE = 6, /* r -= A u (r in/out)                            */
+  /* The first smoothing step fused with the Chebyshev start. The operator
+   * is applied to d0 = (a3*dinv)*u, formed from the input u as it is read
+   * (rounded as tg_cheb_init32 rounds it); with u == rhs this is the
+   * start x = d0, r = rhs of smoothing A x = rhs from zero, and the
+   * outputs equal tg_cheb_init32 followed by CHEB_R (a3 = 0) bit for bit.
+   * Outputs must be distinct and differ from u and rhs.                  */
+  TG_EP_CHEB_FIRST = 7, /* r = rhs - A d0; d = a1*dinv*r + a2*d0;
+                           x = d0 + d                                       */
+  TG_EP_RESID_FIRST = 8, /* degree-1 smoothing: r = rhs - A d0; x = d0
+                           (x is zero at fixed DOFs)                        */
+  TG_EP_CHEB_LAST = 9   /* last step of a smoothing: x as CHEB_R computes
+                           it; r and d are not written                      */
 };
 
 typedef struct {

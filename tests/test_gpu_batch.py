@@ -40,9 +40,10 @@ def _init(gpu):
 
 
 def _ops(dims, rng):
-    """(name, op, n) for every operator kind: fine-grid hex operators on dims,
-    Galerkin level-1 operators (implicit Q8 and stored forms) on the coarse
-    grid. Returns the tensors that must outlive the launches as well."""
+    """(name, op, n, free) for every operator kind: fine-grid hex operators on
+    dims, Galerkin level-1 operators (implicit Q8 and stored forms) on the
+    coarse grid; free is the host bool array of free DOFs. Returns the
+    tensors that must outlive the launches as well."""
     nn = int(np.prod([d + 1 for d in dims]))
     mask = np.full(nn, 7, np.uint8)
     mask.reshape(dims[0] + 1, -1)[0] = 0                       # x = 0 face clamped
@@ -51,8 +52,9 @@ def _ops(dims, rng):
     keep = [t(mask), t(E), t(E.astype(np.float32))]
     dm, dE64, dE32 = keep
     out = []
+    free0, free1 = _free(mask), None
     for kind in (L.TG_OP_HEX_F32, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16X3):
-        out.append((f"hex{kind}", L.TgOp32(kind, L.grid(dims), L.ptr(dm), L.ptr(dE32), None), 3 * nn))
+        out.append((f"hex{kind}", L.TgOp32(kind, L.grid(dims), L.ptr(dm), L.ptr(dE32), None), 3 * nn, free0))
     dc = fem.coarse_dims(dims)
     nc = int(np.prod([d + 1 for d in dc]))
     m1 = fem.injection_mask(dims, dc, mask)
@@ -66,11 +68,19 @@ def _ops(dims, rng):
     coef = torch.zeros(243 * nc, **F32)
     L.call("tg_assemble_stencil", L.grid(dc), L.ptr(dm1), L.ptr(m64), L.ptr(coef), stream())
     keep += [dm1, m64, m32, coef]
+    free1 = _free(m1)
     for kind in (L.TG_OP_Q8_F32, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16X3):
-        out.append((f"q8_{kind}", L.TgOp32(kind, L.grid(dc), L.ptr(dm1), L.ptr(dE32), None), 3 * nc))
-    out.append(("stored", L.TgOp32(L.TG_OP_STORED_F32, L.grid(dc), L.ptr(dm1), None, L.ptr(m32)), 3 * nc))
-    out.append(("stencil", L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dc), L.ptr(dm1), None, None, L.ptr(coef)), 3 * nc))
+        out.append((f"q8_{kind}", L.TgOp32(kind, L.grid(dc), L.ptr(dm1), L.ptr(dE32), None), 3 * nc, free1))
+    out.append(("stored", L.TgOp32(L.TG_OP_STORED_F32, L.grid(dc), L.ptr(dm1), None, L.ptr(m32)), 3 * nc, free1))
+    out.append(("stencil", L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dc), L.ptr(dm1), None, None, L.ptr(coef)), 3 * nc,
+                free1))
     return out, keep
+
+
+def _free(mask):
+    """Per-DOF free flags of a per-node mask (bit c = component c free)."""
+    m = np.asarray(mask, np.uint8)
+    return ((m[:, None] >> np.arange(3)) & 1).reshape(-1).astype(bool)
 
 
 # epilogue -> (buffers it reads or writes besides u; writes y)
@@ -82,6 +92,9 @@ EPS = {
     L.TG_EP_CHEB: (("rhs", "d", "x"), False),
     L.TG_EP_CHEB_R: (("d", "x", "r"), False),
     L.TG_EP_RESID_UPDATE: (("r",), False),
+    L.TG_EP_CHEB_FIRST: (("rhs", "d", "x", "r"), False),
+    L.TG_EP_RESID_FIRST: (("rhs", "x", "r"), False),
+    L.TG_EP_CHEB_LAST: (("d", "x", "r"), False),
 }
 
 
@@ -89,7 +102,7 @@ EPS = {
 def test_batched_operators_bit_identical_to_single(dims):
     rng = np.random.default_rng(11)
     ops, keep = _ops(dims, rng)
-    for name, op, n in ops:
+    for name, op, n, _ in ops:
         dinv = t(rng.uniform(0.5, 2.0, n).astype(np.float32))
         for mode, (names, writes_y) in EPS.items():
             host = {k: rng.standard_normal(NR * n).astype(np.float32) for k in ("u", "rhs", "d", "x", "r")}
@@ -322,3 +335,52 @@ def test_multi_load_solver_api_runs():
     assert np.all(np.isfinite(comp)) and comp[-1] < comp[0]
     assert all(r["converged"] for r in res.history)
     assert res.rho_final.shape == (16 * 8 * 8,)
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 10), (6, 4, 14)])
+def test_fused_smoother_modes_equal_unfused_sequences(dims):
+    """The fused first and last smoothing steps return the bits of the
+    sequences they replace, for every operator kind and a batch of NR:
+    CHEB_FIRST(rhs) = cheb_init(rhs) then CHEB_R(d0) (a3 = 0);
+    RESID_FIRST(rhs) = cheb_init(rhs) then RESID_UPDATE(d0);
+    CHEB_LAST = CHEB_R restricted to x.
+    The right-hand side is zero at fixed DOFs and dinv is 1 there, as in the
+    smoother, so whole vectors compare equal."""
+    rng = np.random.default_rng(5)
+    ops, keep = _ops(dims, rng)
+    a0, a1, a2 = 0.61, 1.37, 0.29
+    for name, op, n, free in ops:
+        dinv_h = np.where(free, rng.uniform(0.5, 2.0, n), 1.0).astype(np.float32)
+        dinv = t(dinv_h)
+        rhs_h = np.tile(np.where(free, 1.0, 0.0), NR).astype(np.float32) * rng.standard_normal(NR * n).astype(np.float32)
+        rhs = t(rhs_h)
+        z = lambda: torch.zeros(NR * n, **F32)
+        # unfused: start, then the first Chebyshev step / the residual update
+        d0, xu, ru, du = z(), z(), z(), z()
+        L.call("tg_cheb_init32_b", n, NR, L.ptr(rhs), L.ptr(dinv), a0, L.ptr(d0), L.ptr(xu), L.ptr(ru), stream())
+        r1 = ru.clone()
+        L.call("tg_apply32_b", op, NR, L.ptr(d0), None,
+               L.ep(L.TG_EP_CHEB_R, dinv=dinv, d=du, x=xu, r=ru, a1=a1, a2=a2), stream())
+        L.call("tg_apply32_b", op, NR, L.ptr(d0), None, L.ep(L.TG_EP_RESID_UPDATE, r=r1), stream())
+        # fused
+        xf, rf, df = z(), z(), z()
+        L.call("tg_apply32_b", op, NR, L.ptr(rhs), None,
+               L.ep(L.TG_EP_CHEB_FIRST, rhs=rhs, dinv=dinv, d=df, x=xf, r=rf, a1=a1, a2=a2, a3=a0), stream())
+        for a, b, what in ((rf, ru, "r"), (df, du, "d"), (xf, xu, "x")):
+            assert np.array_equal(h(a), h(b)), (name, "CHEB_FIRST", what)
+        x1, r2 = z(), z()
+        L.call("tg_apply32_b", op, NR, L.ptr(rhs), None,
+               L.ep(L.TG_EP_RESID_FIRST, rhs=rhs, dinv=dinv, x=x1, r=r2, a3=a0), stream())
+        assert np.array_equal(h(r2), h(r1)), (name, "RESID_FIRST", "r")
+        assert np.array_equal(h(x1), h(d0)), (name, "RESID_FIRST", "x")
+        # last step: CHEB_R from the state just produced vs CHEB_LAST
+        xr, rr, dr = xu.clone(), ru.clone(), z()
+        xl, rl, dl = xu.clone(), ru.clone(), z()
+        L.call("tg_apply32_b", op, NR, L.ptr(du), None,
+               L.ep(L.TG_EP_CHEB_R, dinv=dinv, d=dr, x=xr, r=rr, a1=a1, a2=a2, a3=0.5), stream())
+        L.call("tg_apply32_b", op, NR, L.ptr(du), None,
+               L.ep(L.TG_EP_CHEB_LAST, dinv=dinv, d=dl, x=xl, r=rl, a1=a1, a2=a2, a3=0.5), stream())
+        assert np.array_equal(h(xl), h(xr)), (name, "CHEB_LAST", "x")
+        assert np.array_equal(h(rl), h(ru)) and not h(dl).any(), (name, "CHEB_LAST wrote r or d")
+    del keep
+

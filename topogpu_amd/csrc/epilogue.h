@@ -50,6 +50,8 @@ struct EpMode {
     float p[3], q[3], s[3];
   };
   static constexpr bool kReduces = MODE == TG_EP_DOT || MODE == TG_EP_RESID_NORM;
+  // the operator's input is d0 = a3 * dinv * u, formed as it is read
+  static constexpr bool kPrescale = MODE == TG_EP_CHEB_FIRST || MODE == TG_EP_RESID_FIRST;
 
   // One component c of a node: fetch the mode's inputs.
   template <class Acc>
@@ -61,12 +63,17 @@ struct EpMode {
       p = a.template ld<EV_RHS>(c);
       q = a.template ld<EV_DINV>(c);
       if (e.a2 != 0.0f) s = a.template ld<EV_D>(c);
-    } else if constexpr (MODE == TG_EP_CHEB_R) {
+    } else if constexpr (MODE == TG_EP_CHEB_R || MODE == TG_EP_CHEB_LAST) {
       p = a.template ld<EV_R>(c);
       q = a.template ld<EV_DINV>(c);
       s = a.template ld<EV_X>(c);
     } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
       p = a.template ld<EV_R>(c);
+    } else if constexpr (MODE == TG_EP_CHEB_FIRST) {
+      p = a.template ld<EV_RHS>(c);
+      q = a.template ld<EV_DINV>(c);
+    } else if constexpr (MODE == TG_EP_RESID_FIRST) {
+      p = a.template ld<EV_RHS>(c);
     }
   }
 
@@ -101,6 +108,20 @@ struct EpMode {
       a.template st<EV_X>(c, f ? s + e.a3 * u + dn : 0.0f);
     } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
       a.template st<EV_R>(c, f ? p - au : 0.0f);
+    } else if constexpr (MODE == TG_EP_CHEB_FIRST) {
+      // u is d0 here: CHEB_R's arithmetic from the start x = d0, r = rhs
+      const float rn = p - au;
+      const float dn = e.a1 * q * rn + e.a2 * u;
+      a.template st<EV_R>(c, f ? rn : 0.0f);
+      a.template st<EV_D>(c, f ? dn : 0.0f);
+      a.template st<EV_X>(c, f ? u + dn : 0.0f);
+    } else if constexpr (MODE == TG_EP_RESID_FIRST) {
+      a.template st<EV_R>(c, f ? p - au : 0.0f);
+      a.template st<EV_X>(c, f ? u : 0.0f);
+    } else if constexpr (MODE == TG_EP_CHEB_LAST) {
+      const float rn = p - au;
+      const float dn = e.a1 * q * rn + e.a2 * u;
+      a.template st<EV_X>(c, f ? s + e.a3 * u + dn : 0.0f);
     }
     return part;
   }
@@ -146,6 +167,7 @@ struct Ep32M : Ep32 {
   using Mode = EpMode<MODE>;
   using In = typename Mode::In;
   static constexpr int kMode = MODE;
+  static constexpr bool kPrescale = Mode::kPrescale;
   __device__ __forceinline__ void load(long long node, In& in) const { Mode::load(*this, PtrAcc{*this, 3 * node}, in); }
   __device__ __forceinline__ double store(long long node, unsigned m, const float (&au)[3], const float (&u)[3],
                                           const In& in) const {
@@ -173,6 +195,9 @@ int with_mode(const Ep32& e, F&& f) {
     case TG_EP_CHEB: return f(Ep32M<TG_EP_CHEB>{e});
     case TG_EP_CHEB_R: return f(Ep32M<TG_EP_CHEB_R>{e});
     case TG_EP_RESID_UPDATE: return f(Ep32M<TG_EP_RESID_UPDATE>{e});
+    case TG_EP_CHEB_FIRST: return f(Ep32M<TG_EP_CHEB_FIRST>{e});
+    case TG_EP_RESID_FIRST: return f(Ep32M<TG_EP_RESID_FIRST>{e});
+    case TG_EP_CHEB_LAST: return f(Ep32M<TG_EP_CHEB_LAST>{e});
     default: return set_error(TG_ERR_ARG, "unknown epilogue mode %d", e.mode);
   }
 }
@@ -184,6 +209,7 @@ struct Ep64 {
   double* y;
   float* y32;
   bool norm;
+  static constexpr bool kPrescale = false;
 
   __device__ __forceinline__ Ep64 slice(long long off) const {
     Ep64 e = *this;

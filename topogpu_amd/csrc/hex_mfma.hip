@@ -197,9 +197,12 @@ __device__ __forceinline__ Block block_setup(const Grid& g, TileShape ts, const 
   return tile_block<T>(g, ts, xcd_remap(blockIdx.x, gridDim.x), mask, u + roff, ep_slice(ep_all, roff));
 }
 
-// Node brick loads: every load issued before any is used.
-template <class T>
-__device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT][3], uint32_t (&bm)[T::NIT]) {
+// Node brick loads: every load issued before any is used. PRE also loads
+// dinv at the same nodes (bd) for the prescaled input of the fused first
+// smoothing step.
+template <class T, bool PRE>
+__device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT][3], uint32_t (&bm)[T::NIT],
+                                           uint32_t (&bd)[T::NIT][3]) {
   const int t = threadIdx.x, bz = t % T::UBZ, l0 = t / T::UBZ, bx0 = l0 / T::UBY, by = l0 % T::UBY;
   const int i0 = b.i0 - 1 + bx0, j = b.j0 - 1 + by, k = b.k0 - 1 + bz;
   const bool ok = (t < T::BACT) & ((unsigned)j < (unsigned)b.NYn) & ((unsigned)k < (unsigned)b.NZn);
@@ -214,6 +217,12 @@ __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT
     bw[it][0] = w3[0];
     bw[it][1] = w3[1];
     bw[it][2] = w3[2];
+    if constexpr (PRE) {
+      const auto d3 = __builtin_amdgcn_raw_buffer_load_b96(b.acc.dinv, oob_unless(12u * rel, in), 0, 0);
+      bd[it][0] = d3[0];
+      bd[it][1] = d3[1];
+      bd[it][2] = d3[2];
+    }
   }
 }
 
@@ -221,16 +230,23 @@ __device__ __forceinline__ void brick_load(const Block& b, uint32_t (&bw)[T::NIT
 // as zero). Every thread stores unconditionally -- idle threads into a dump
 // slot of their own past the brick -- because a conditional store would let
 // the compiler sink the matching loads into the branch and wait there.
-template <class T>
-__device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::NIT][3], const uint32_t (&bm)[T::NIT]) {
+// PRE: the stored value is (pa * dinv) * u, rounded as the separate
+// Chebyshev start rounds it.
+template <class T, bool PRE>
+__device__ __forceinline__ void brick_store(float4* su, const uint32_t (&bw)[T::NIT][3], const uint32_t (&bm)[T::NIT],
+                                            const uint32_t (&bd)[T::NIT][3], float pa) {
   const int t = threadIdx.x;
   const bool act = t < T::BACT;
 #pragma unroll
   for (int it = 0; it < T::NIT; ++it) {
     const int idx = act ? t + it * T::BACT : T::NIT * T::BACT + (t - T::BACT);
-    su[idx] = make_float4((bm[it] & 1u) ? __builtin_bit_cast(float, bw[it][0]) : 0.f,
-                          (bm[it] & 2u) ? __builtin_bit_cast(float, bw[it][1]) : 0.f,
-                          (bm[it] & 4u) ? __builtin_bit_cast(float, bw[it][2]) : 0.f, 0.f);
+    float v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c] = __builtin_bit_cast(float, bw[it][c]);
+      if constexpr (PRE) v[c] = (pa * __builtin_bit_cast(float, bd[it][c])) * v[c];
+    }
+    su[idx] = make_float4((bm[it] & 1u) ? v[0] : 0.f, (bm[it] & 2u) ? v[1] : 0.f, (bm[it] & 4u) ? v[2] : 0.f, 0.f);
   }
 }
 
@@ -411,14 +427,14 @@ __global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, 
   // loads in the order they are needed (they complete in issue order)
   bf16x8 ah[2][2], al[2][2];
   ke_frags<SPLIT>(pl.lane, ah, al);
-  uint32_t bw[T::NIT][3], bm[T::NIT];
-  brick_load<T>(b, bw, bm);
+  uint32_t bw[T::NIT][3], bm[T::NIT], bd[T::NIT][3];
+  brick_load<T, Mode::kPrescale>(b, bw, bm, bd);
   float ev[T::PY];
   e_load<T>(g, b, E, w, pl.je, ev);
   uint32_t ob[T::OIT], om[T::OIT];
   typename Mode::In oin[T::OIT];
   out_prefetch<T, MODE>(b, ep, ob, om, oin);
-  brick_store<T>(su, bw, bm);
+  brick_store<T, Mode::kPrescale>(su, bw, bm, bd, ep.a3);
   __syncthreads();
   pencils0<SPLIT, T>(su, slot, pl, ah, al, ev);
   __syncthreads();
@@ -452,8 +468,8 @@ __global__ void __launch_bounds__(Tile1::NTHR) k_q8_mfma(Grid g, TileShape ts, c
   const long long fslab = 2ll * b.is * fny * fnz, fne = 8ll * g.nx * g.ny * g.nz;
   const rsrc_t rs_e = make_rsrc(Ef + fslab, 4ull * (fne - fslab));
 
-  uint32_t bw[T::NIT][3], bm[T::NIT];
-  brick_load<T>(b, bw, bm);
+  uint32_t bw[T::NIT][3], bm[T::NIT], bd[T::NIT][3];
+  brick_load<T, Mode::kPrescale>(b, bw, bm, bd);
   uint32_t ob[T::OIT], om[T::OIT];
   typename Mode::In oin[T::OIT];
   out_prefetch<T, MODE>(b, ep, ob, om, oin);
@@ -469,7 +485,7 @@ __global__ void __launch_bounds__(Tile1::NTHR) k_q8_mfma(Grid g, TileShape ts, c
     const uint32_t off = fbase + (uint32_t)(((s >> 2) * fny + 2 * py + ((s >> 1) & 1)) * fnz + (s & 1));
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_e, oob_unless(4u * off, ok), 0, 0));
   };
-  brick_store<T>(su, bw, bm);
+  brick_store<T, Mode::kPrescale>(su, bw, bm, bd, ep.a3);
   __syncthreads();
 
   float rp[3] = {0.f, 0.f, 0.f};

@@ -69,10 +69,12 @@ struct Q8W {
   __device__ const float* mat(int s) const { return c_q8_32 + 576 * s; }
 };
 
-template <typename T>
+// PRE: the brick holds (pa * dinv) * u instead of u (the Chebyshev start
+// d0 of the fused first smoothing step, rounded as the separate start is).
+template <typename T, bool PRE>
 __device__ __forceinline__ void load_node_brick(const Grid& g, const uint8_t* __restrict__ mask,
                                                 const T* __restrict__ u, int i0, int j0, int k0,
-                                                T* su) {
+                                                T* su, const float* __restrict__ pd, float pa) {
   // All loads first (none depends on another), then the masked LDS stores.
   constexpr int NB = BX * BY * BZ, NIT = (NB + NT - 1) / NT;
   T v[NIT][3];
@@ -86,7 +88,10 @@ __device__ __forceinline__ void load_node_brick(const Grid& g, const uint8_t* __
     const long long n = in ? g.node(i, j, k) : 0;
     vm[it] = in ? (unsigned)mask[n] : 0u;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[it][c] = in ? u[3 * n + c] : T(0);
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (PRE) v[it][c] = in ? (pa * pd[3 * n + c]) * u[3 * n + c] : T(0);
+      else v[it][c] = in ? u[3 * n + c] : T(0);
+    }
   }
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
@@ -111,7 +116,8 @@ __global__ void __launch_bounds__(NT) k_node_op(Grid g, const uint8_t* __restric
   const Ep ep = ep_all.slice(roff);
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
-  load_node_brick<T>(g, mask, u, i0, j0, k0, su);
+  if constexpr (Ep::kPrescale) load_node_brick<T, true>(g, mask, u, i0, j0, k0, su, ep.dinv, ep.a3);
+  else load_node_brick<T, false>(g, mask, u, i0, j0, k0, su, nullptr, 0.0f);
   for (int idx = threadIdx.x; idx < EX * EY * EZ; idx += blockDim.x) {
     const int ez = idx % EZ, ey = (idx / EZ) % EY, ex = idx / (EZ * EY);
     const int i = i0 - 1 + ex, j = j0 - 1 + ey, k = k0 - 1 + ez;
@@ -227,7 +233,8 @@ __global__ void __launch_bounds__(NT) k_node_stored(Grid g, const uint8_t* __res
   const Ep ep = ep_all.slice(roff);
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
-  load_node_brick<float>(g, mask, u, i0, j0, k0, su);
+  if constexpr (Ep::kPrescale) load_node_brick<float, true>(g, mask, u, i0, j0, k0, su, ep.dinv, ep.a3);
+  else load_node_brick<float, false>(g, mask, u, i0, j0, k0, su, nullptr, 0.0f);
   __syncthreads();
   const int tz = threadIdx.x % NTZ, ty = (threadIdx.x / NTZ) % NTY, tx = threadIdx.x / (NTZ * NTY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
@@ -351,6 +358,21 @@ extern "C" int tg_apply32_b(const tg_op32* op, int32_t nrhs, const float* u, flo
       break;
     case TG_EP_RESID_UPDATE:
       TG_REQUIRE(ep->r && ep->r != u, "RESID_UPDATE needs r distinct from the input");
+      break;
+    case TG_EP_CHEB_FIRST:
+      TG_REQUIRE(ep->rhs && ep->dinv && ep->d && ep->x && ep->r, "CHEB_FIRST needs rhs, dinv, d, x, r");
+      TG_REQUIRE(ep->d != u && ep->r != u && ep->x != u && ep->d != ep->rhs && ep->r != ep->rhs &&
+                     ep->x != ep->rhs && ep->d != ep->x && ep->d != ep->r && ep->x != ep->r,
+                 "CHEB_FIRST outputs must be distinct and differ from the input and rhs");
+      break;
+    case TG_EP_RESID_FIRST:
+      TG_REQUIRE(ep->rhs && ep->dinv && ep->x && ep->r, "RESID_FIRST needs rhs, dinv, x, r");
+      TG_REQUIRE(ep->r != u && ep->x != u && ep->r != ep->rhs && ep->x != ep->rhs && ep->x != ep->r,
+                 "RESID_FIRST outputs must be distinct and differ from the input and rhs");
+      break;
+    case TG_EP_CHEB_LAST:
+      TG_REQUIRE(ep->r && ep->dinv && ep->x, "CHEB_LAST needs r, dinv, x");
+      TG_REQUIRE(ep->x != u && ep->x != ep->r, "CHEB_LAST x must differ from the input and r");
       break;
     default: return set_error(TG_ERR_ARG, "unknown epilogue mode %d", ep->mode);
   }

@@ -81,18 +81,23 @@ __global__ void __launch_bounds__(SNT) k_stencil_apply(Grid g, const uint8_t* __
   Mode::load_one(ep, acc, c, p, q, s);
   const long long sz = g.NZ(), sy = (long long)g.NZ() * g.NY();
   const float* cf = coef + (long long)(c * 81) * nn + nc;
+  // the operator's input (prescaled modes: the Chebyshev start a3 * dinv * u)
+  auto in_at = [&](long long i) -> float {
+    if constexpr (Mode::kPrescale) return (ep.a3 * ep.dinv[i]) * u[i];
+    else return u[i];
+  };
   float au = 0.0f;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     long long nb = nc + (k / 9 - 1) * sy + ((k / 3) % 3 - 1) * sz + (k % 3 - 1);
     nb = nb < 0 ? 0 : (nb >= nn ? nn - 1 : nb);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) au = fmaf(cf[(long long)(3 * k + d) * nn], u[3 * nb + d], au);
+    for (int d = 0; d < 3; ++d) au = fmaf(cf[(long long)(3 * k + d) * nn], in_at(3 * nb + d), au);
   }
   double part = 0.0;
   if (own) {
     const bool f = mask[n] >> c & 1u;
-    const float uo = f ? u[3 * n + c] : 0.0f;
+    const float uo = f ? in_at(3 * n + c) : 0.0f;
     part = Mode::store_one(ep, acc, c, f, au, uo, p, q, s);
   }
   if constexpr (Mode::kReduces) {

@@ -47,7 +47,8 @@ KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_Q8_BF16: "galerkin_q8_bf16_mfma", L.TG_OP_Q8_BF16X3: "galerkin_q8_bf16x3_mfma"}
 
 _EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
-                   L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8}
+                   L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8, L.TG_EP_CHEB_FIRST: 16, L.TG_EP_RESID_FIRST: 12,
+                   L.TG_EP_CHEB_LAST: 16}
 
 
 def hex_bytes(n_dof: int, n_elem: int, mode: int) -> float:
@@ -264,29 +265,41 @@ class GMG:
         return out
 
     def _presmooth(self, lv: int, rhs, degree: int, stream):
-        """x from zero; leaves r = rhs - A x in Lv.r. Returns the last step buffer."""
+        """x from zero; leaves r = rhs - A x in Lv.r.
+
+        The Chebyshev start d0 = alpha0 Dinv rhs (x = d0) is not written
+        out: the first operator application forms it from rhs as it reads
+        its input (CHEB_FIRST, or RESID_FIRST for degree 1), with the same
+        rounding as a separate start, so the results are identical."""
         Lv = self.levels[lv]
         co = self._cheb_coeffs(lv, degree)
-        L.call("tg_cheb_init32_b", Lv.n, self.nrhs, L.ptr(rhs), L.ptr(Lv.dinv), co[0][0], L.ptr(Lv.da), L.ptr(Lv.x),
-               L.ptr(Lv.r), stream)
-        d, dn = Lv.da, Lv.db
         op = self.op(lv)
-        for a1, a2 in co[1:]:
+        if degree == 1:
+            self._launch(lv, op, rhs, None,
+                         L.ep(L.TG_EP_RESID_FIRST, rhs=rhs, dinv=Lv.dinv, x=Lv.x, r=Lv.r, a3=co[0][0]), stream)
+            return
+        a1, a2 = co[1]
+        self._launch(lv, op, rhs, None, L.ep(L.TG_EP_CHEB_FIRST, rhs=rhs, dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r,
+                                             a1=a1, a2=a2, a3=co[0][0]), stream)
+        d, dn = Lv.da, Lv.db
+        for a1, a2 in co[2:]:
             self._launch(lv, op, d, None, L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
             d, dn = dn, d
         self._launch(lv, op, d, None, L.ep(L.TG_EP_RESID_UPDATE, r=Lv.r), stream)
 
     def _postsmooth(self, lv: int, degree: int, stream):
-        """x += c (the prolonged correction, in Lv.c) then Chebyshev from there."""
+        """x += c (the prolonged correction, in Lv.c) then Chebyshev from there.
+        The last step writes only x (CHEB_LAST): r and d are dead after it."""
         Lv = self.levels[lv]
         co = self._cheb_coeffs(lv, degree)
+        last = lambda i: L.TG_EP_CHEB_LAST if i == degree - 1 else L.TG_EP_CHEB_R
         # r -= A c ; d1 = alpha0 Dinv r ; x += c + d1   (accurate operator: c is smooth)
         self._launch(lv, self.op(lv, accurate=True), Lv.c, None,
-                     L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r, a1=co[0][0], a2=0.0, a3=1.0), stream)
+                     L.ep(last(0), dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r, a1=co[0][0], a2=0.0, a3=1.0), stream)
         d, dn = Lv.da, Lv.db
         op = self.op(lv)
-        for a1, a2 in co[1:]:
-            self._launch(lv, op, d, None, L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
+        for i, (a1, a2) in enumerate(co[1:], start=1):
+            self._launch(lv, op, d, None, L.ep(last(i), dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
             d, dn = dn, d
 
     def _vcycle(self, lv: int, rhs, stream):

@@ -15,7 +15,8 @@ LIB_PATH = Path(__file__).resolve().parent.parent / "libtopohip.so"
 
 TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32, TG_OP_STENCIL_F32 = 1, 2, 3, 4, 5, 6
 TG_OP_Q8_BF16, TG_OP_Q8_BF16X3 = 7, 8
-TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE = range(7)
+(TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE,
+ TG_EP_CHEB_FIRST, TG_EP_RESID_FIRST, TG_EP_CHEB_LAST) = range(10)
 OC_STATE_LEN = 32   # device doubles of the OC bisection state (TG_OC_STATE_LEN)
 OC_MEAN_STATE_LEN = 80   # ... of the mean-volume OC (TG_OC_MEAN_STATE_LEN)
 
