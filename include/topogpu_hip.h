@@ -257,6 +257,23 @@ int tg_fill64(int64_t n, double* x, double value, void* stream);
 /* dst = src (n floats): p = z of _cupy_pcg (solver_v2.py:603) as a kernel. */
 int tg_copy32(int64_t n, float* dst, const float* src, void* stream);
 
+/* ---- replayable launch sequences (HIP graphs) ----------------------------
+ * No counterpart in the reference, whose PCG loop (_cupy_pcg,
+ * src/gpu_fem/solver_v2.py:552, called from solver_v4.py:193) issues every
+ * kernel from the host on each iteration. Here one Krylov iteration (V-cycle
+ * plus the PCG updates, ~50 launches) is captured once and replayed with a
+ * single call, which removes the host's per-launch cost from the loop.
+ * tg_graph_begin starts capturing the launches issued on `stream` (a created
+ * stream, not the null stream); nothing runs while capturing. tg_graph_end
+ * ends the capture and instantiates it into *exec. tg_graph_launch replays
+ * it on any stream. Captured launches keep the arguments they had at capture
+ * (pointers and scalars): a caller re-captures when one of them changes.
+ * Captured sequences hold kernel launches only (no memset nodes).          */
+int tg_graph_begin(void* stream);
+int tg_graph_end(void* stream, void** exec);
+int tg_graph_launch(void* exec, void* stream);
+int tg_graph_destroy(void* exec);
+
 /* ---- SIMP density / sensitivity / OC (FP64) ----------------------------- */
 
 /* out[i] = (sum_k w_k * s(in)[i+off_k]) * (out_scale ? out_scale[i] : 1),

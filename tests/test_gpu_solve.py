@@ -99,3 +99,34 @@ def test_solverv4_formulation_matches_oracle_loop(gpu):
     assert np.abs(res.rho_final - rho).max() < 1e-6
     assert res.summary["n_iter"] == 4 and len(res.history) == 4
     assert res.history[-1]["rho_mean"] == pytest.approx(0.30, abs=1e-8)
+
+
+@pytest.mark.parametrize("nrhs", [1, 2])
+def test_graph_replay_equals_direct_launches(gpu, nrhs):
+    """The Krylov iterations replayed from captured graphs give the bits of
+    the same iterations launched one by one: over three solves whose
+    modulus changes (the second by more than 10 %, which re-estimates the
+    smoother's lambda and forces a new capture), each replayed many times
+    with host reads of the residual between replays."""
+    dims = (16, 8, 8)
+    free, F, ndof = cantilever_bc(dims, point_support=True)
+    rng = np.random.default_rng(4)
+    Fs = np.stack([F] + [rng.standard_normal(ndof) * (np.abs(F).max()) for _ in range(nrhs - 1)])
+    Es = [1e-9 + rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3 for _ in range(3)]
+    Es[1] = 1.5 * Es[1]
+    out = {}
+    for graphs in (False, True):
+        s = HipStateSolver(dims, free, Fs, precision="bf16", n_levels=3, tol=1e-8, max_krylov=400, graphs=graphs)
+        res = []
+        for E in Es:
+            E64 = torch.from_numpy(E).to(DEV)
+            s.set_modulus(E64, E64.float(), float(E.max()))
+            st = s.solve()
+            assert st.converged, st
+            res.append((st.iters, st.restarts, h(s.u).copy()))
+        out[graphs] = res
+        if graphs:
+            assert len(s._graphs) == 2, "first and later iterations are captured"
+    for (i0, r0, u0), (i1, r1, u1) in zip(out[False], out[True]):
+        assert (i0, r0) == (i1, r1)
+        assert np.array_equal(u0, u1)

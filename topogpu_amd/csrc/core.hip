@@ -346,6 +346,32 @@ extern "C" int tg_fill32(int64_t n, float* x, float v, void* s) {
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
+extern "C" int tg_graph_begin(void* s) {
+  TG_REQUIRE(s, "tg_graph_begin: capture needs a created stream, not the null stream");
+  TG_HIP_CHECK(hipStreamBeginCapture(TG_STREAM(s), hipStreamCaptureModeThreadLocal));
+  return TG_OK;
+}
+extern "C" int tg_graph_end(void* s, void** exec) {
+  TG_REQUIRE(s && exec, "tg_graph_end: NULL argument");
+  *exec = nullptr;
+  hipGraph_t g = nullptr;
+  TG_HIP_CHECK(hipStreamEndCapture(TG_STREAM(s), &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (rc != hipSuccess) return set_error(TG_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(rc));
+  *exec = e;
+  return TG_OK;
+}
+extern "C" int tg_graph_launch(void* exec, void* s) {
+  TG_REQUIRE(exec, "tg_graph_launch: NULL graph");
+  TG_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)exec, TG_STREAM(s)));
+  return TG_OK;
+}
+extern "C" int tg_graph_destroy(void* exec) {
+  if (exec) TG_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)exec));
+  return TG_OK;
+}
 extern "C" int tg_copy32(int64_t n, float* dst, const float* src, void* s) {
   TG_REQUIRE(n >= 0 && (n == 0 || (dst && src)), "bad copy32 args");
   if (n == 0) return TG_OK;

@@ -317,6 +317,14 @@ class GMG:
         self._postsmooth(lv, deg, stream)
         return Lv.x
 
+    def graph_key(self) -> tuple:
+        """What a captured V-cycle bakes in besides the level buffers made at
+        construction: the smoother coefficients (from each level's lambda),
+        the modulus buffers and the setup-time buffers."""
+        return (tuple(float(Lv.lam) for Lv in self.levels), self.fine_degree, tuple(self.coarse_degrees),
+                self.safety, self.frac, L.ptr(self.E32), L.ptr(self.E64),
+                tuple(L.ptr(Lv.stencil) for Lv in self.levels), L.ptr(self.coarse_A), self.nrhs)
+
     def apply(self, r, stream):
         """z = M r for every right-hand side of the batch (r: nrhs slices of
         the level-0 length); returns a level-0 buffer valid until the next

@@ -51,6 +51,10 @@ _SIGS = {
     "tg_last_error": None,
     "tg_version": [],
     "tg_apply32": [C.POINTER(TgOp32), _vp, _vp, C.POINTER(TgEp32), _vp],
+    "tg_graph_begin": [_vp],
+    "tg_graph_end": [_vp, C.POINTER(_vp)],
+    "tg_graph_launch": [_vp, _vp],
+    "tg_graph_destroy": [_vp],
     "tg_apply64": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_mfma_probe": [_vp, _vp, _vp, _vp],
     "tg_diag_inv_hex": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp],

@@ -28,6 +28,7 @@ import numpy as np
 import torch
 
 from .. import fem
+from . import graph
 from . import lib as L
 from .gmg import GMG, launch_op
 
@@ -77,7 +78,7 @@ class HipStateSolver:
                  fine_degree: int = 2, coarse_degree=(1, 4), cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
-                 warm_start: bool = True, level1: str = "q8", device=None):
+                 warm_start: bool = True, level1: str = "q8", graphs: bool = True, device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -126,6 +127,9 @@ class HipStateSolver:
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
+        # one Krylov iteration as a captured launch sequence (first / later)
+        self.graphs = bool(graphs)
+        self._graphs: dict = {}
 
     @property
     def stream(self):
@@ -168,35 +172,65 @@ class HipStateSolver:
                 L.call("tg_fill64", n, L.ptr(self._slice(self.u, k)), 0.0, st)
             rnorm = self.residual64()
             restarts += 1
-        op = self.fine_op()
         s = self.s
         while np.any(rnorm > target) and its < self.max_krylov and restarts <= self.max_restarts:
             inner_target = np.maximum(self.inner_rtol * rnorm, 0.5 * target)
-            L.call("tg_fill32", nr * n, L.ptr(self.d), 0.0, st)
-            z = self.gmg.apply(self.r, st)
+            self._iteration(True, st)
             its += 1
-            L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0]), None, st)
-            L.call("tg_copy32", nr * n, L.ptr(self.p), L.ptr(z), st)
             while True:
-                ep = L.ep(L.TG_EP_DOT, dot=s[1])
-                launch_op(self.gmg.timer, 0, op, L.ptr(self.p), L.ptr(self.q), ep, st, n, self.n_elem, nr)
-                L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
-                       L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
                 rr = s[2].cpu().numpy()
                 if not np.all(np.isfinite(rr)):
                     raise FloatingPointError("non-finite inner residual")
                 if np.all(np.sqrt(np.maximum(rr, 0.0)) <= inner_target) or its >= self.max_krylov:
                     break
-                z = self.gmg.apply(self.r, st)
+                self._iteration(False, st)
                 its += 1
-                L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), L.ptr(self.q), L.ptr(s[0]), L.ptr(s[3]), st)
-                L.call("tg_pcg_dir_b", n, nr, L.ptr(self.p), L.ptr(z), L.ptr(s[3]), L.ptr(s[1]), st)
             L.call("tg_axpy_f32_to_f64", nr * n, L.ptr(self.u), L.ptr(self.d), st)
             rnorm = self.residual64()
             restarts += 1
         rel = rnorm / np.maximum(self.bnorms, 1e-300)
         self.last = SolveStats(its, restarts, bool(np.all(rnorm <= target)), float(rel.max()))
         return self.last
+
+    # ------------------------------------------------------------------
+    def _issue_iteration(self, first: bool, st) -> None:
+        """One flexible-PCG iteration on stream st: z = M r, then
+        first: d = 0, p = z;  later: p = z + beta p (beta = -(z.q)/(p.q));
+        then q = A p and x, r updated with alpha = (r.z)/(p.q), rr = r.r.
+        Scalars stay on the device (rows of self.s)."""
+        n, nr, s = self.n, self.nrhs, self.s
+        if first:
+            L.call("tg_fill32", nr * n, L.ptr(self.d), 0.0, st)
+        z = self.gmg.apply(self.r, st)
+        if first:
+            L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0]), None, st)
+            L.call("tg_copy32", nr * n, L.ptr(self.p), L.ptr(z), st)
+        else:
+            L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), L.ptr(self.q), L.ptr(s[0]), L.ptr(s[3]), st)
+            L.call("tg_pcg_dir_b", n, nr, L.ptr(self.p), L.ptr(z), L.ptr(s[3]), L.ptr(s[1]), st)
+        ep = L.ep(L.TG_EP_DOT, dot=s[1])
+        launch_op(self.gmg.timer, 0, self.fine_op(), L.ptr(self.p), L.ptr(self.q), ep, st, n, self.n_elem, nr)
+        L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
+               L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
+
+    def _iteration(self, first: bool, st) -> None:
+        """Issue one iteration: replay its captured graph (captured again when
+        what it baked in changed: the GMG's coefficients and buffers, the
+        modulus buffer), or launch it directly when graphs are off or a
+        timer records per-launch events."""
+        timer = self.gmg.timer
+        if not self.graphs or (timer is not None and timer.events):
+            self._issue_iteration(first, st)
+            return
+        key = (self.gmg.graph_key(), L.ptr(self.E32))
+        g = self._graphs.get(first)
+        if g is None or g.key != key:
+            if g is not None:
+                g.close()
+            g = self._graphs[first] = graph.capture(key, lambda cs: self._issue_iteration(first, cs), self.gmg,
+                                                    self.dev)
+        g.launch(st)
+        graph.account(timer, g)
 
     def compliances(self) -> np.ndarray:
         """F_k . u_k in FP64 for every load case (solver_v4.py:244)."""
