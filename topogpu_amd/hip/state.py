@@ -78,7 +78,7 @@ class HipStateSolver:
                  fine_degree: int = 2, coarse_degree=(1, 4), cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
-                 warm_start: bool = True, level1: str = "q8", graphs: bool = True, device=None):
+                 warm_start: bool = True, level1: str = "q8", graphs: bool = False, device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
