@@ -63,42 +63,59 @@ __global__ void __launch_bounds__(SNT) k_assemble_stencil(Grid g, const uint8_t*
   }
 }
 
+// One workgroup = SAP nodes x 3 waves; wave w sums the 9 neighbours in the
+// x-plane dx = w - 1 (27 coefficient and 27 value loads per lane, all in
+// flight at once: the coarse levels are small and bound by load latency, and
+// a lane with all 81 products would need several dependent rounds of loads).
+// Wave 2 adds the three plane sums in the order dx = -1, 0, 1 and applies the
+// epilogue.
+constexpr int SAP = 64;
+
 template <int MODE>
-__global__ void __launch_bounds__(SNT) k_stencil_apply(Grid g, const uint8_t* __restrict__ mask,
-                                                      const float* __restrict__ coef, const float* __restrict__ u,
-                                                      Ep32 ep_all, Workspace ws) {
+__global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t* __restrict__ mask,
+                                                          const float* __restrict__ coef, const float* __restrict__ u,
+                                                          Ep32 ep_all, Workspace ws) {
   using Mode = EpMode<MODE>;
+  __shared__ float plane_sum[2][SAP];
   const long long nn = g.nodes();
   // right-hand side blockIdx.z of a batch
   u += (long long)blockIdx.z * 3 * nn;
   const Ep32 ep = ep_slice(ep_all, (long long)blockIdx.z * 3 * nn);
-  const long long n = blockIdx.x * (long long)SNT + threadIdx.x;
+  const int lane = threadIdx.x % SAP;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / SAP);
+  const long long n = blockIdx.x * (long long)SAP + lane;
   const int c = blockIdx.y;
   const bool own = n < nn;
   const long long nc = own ? n : nn - 1;
-  float p, q, s;
-  const PtrAcc acc{ep, 3 * nc};
-  Mode::load_one(ep, acc, c, p, q, s);
-  const long long sz = g.NZ(), sy = (long long)g.NZ() * g.NY();
-  const float* cf = coef + (long long)(c * 81) * nn + nc;
   // the operator's input (prescaled modes: the Chebyshev start a3 * dinv * u)
   auto in_at = [&](long long i) -> float {
     if constexpr (Mode::kPrescale) return (ep.a3 * ep.dinv[i]) * u[i];
     else return u[i];
   };
+  float p = 0.0f, q = 0.0f, s = 0.0f, uo = 0.0f;
+  const PtrAcc acc{ep, 3 * nc};
+  if (w == 2) {
+    Mode::load_one(ep, acc, c, p, q, s);
+    uo = in_at(3 * nc + c);
+  }
+  const long long sz = g.NZ(), sy = (long long)g.NZ() * g.NY();
+  const float* cf = coef + (long long)(c * 81 + 27 * w) * nn + nc;
+  const long long base = nc + (w - 1) * sy;
   float au = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    long long nb = nc + (k / 9 - 1) * sy + ((k / 3) % 3 - 1) * sz + (k % 3 - 1);
+  for (int j = 0; j < 9; ++j) {
+    long long nb = base + (j / 3 - 1) * sz + (j % 3 - 1);
     nb = nb < 0 ? 0 : (nb >= nn ? nn - 1 : nb);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) au = fmaf(cf[(long long)(3 * k + d) * nn], in_at(3 * nb + d), au);
+    for (int d = 0; d < 3; ++d) au = fmaf(cf[(long long)(3 * j + d) * nn], in_at(3 * nb + d), au);
   }
+  if (w < 2) plane_sum[w][lane] = au;
+  __syncthreads();
   double part = 0.0;
-  if (own) {
+  if (w == 2 && own) {
+    au = (plane_sum[0][lane] + plane_sum[1][lane]) + au;
     const bool f = mask[n] >> c & 1u;
-    const float uo = f ? in_at(3 * n + c) : 0.0f;
-    part = Mode::store_one(ep, acc, c, f, au, uo, p, q, s);
+    part = Mode::store_one(ep, acc, c, f, au, f ? uo : 0.0f, p, q, s);
   }
   if constexpr (Mode::kReduces) {
     double v[1] = {part};
@@ -111,12 +128,12 @@ __global__ void __launch_bounds__(SNT) k_stencil_apply(Grid g, const uint8_t* __
 int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
                    double* dot, int nrhs, hipStream_t s) {
   const long long nn = g.nodes();
-  const long long nbx = (nn + SNT - 1) / SNT;
+  const long long nbx = (nn + SAP - 1) / SAP;
   TG_REQUIRE(nbx * 3 * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
-    hipLaunchKernelGGL((k_stencil_apply<M>), dim3((unsigned)nbx, 3, nrhs), dim3(SNT), 0, s, g, mask, coef, u, ep,
-                       workspace());
+    hipLaunchKernelGGL((k_stencil_apply<M>), dim3((unsigned)nbx, 3, nrhs), dim3(3 * SAP), 0, s, g, mask, coef, u,
+                       ep, workspace());
     TG_LAUNCH_CHECK();
     return EpMode<M>::kReduces ? finalize_batched(1, (int)(nbx * 3), nrhs, dot, nullptr, s) : TG_OK;
   });
