@@ -426,7 +426,8 @@ def test_not_spd_is_reported():
     A = t(np.array([1.0, 2.0, 2.0, 1.0]))
     with pytest.raises(L.TopoHipError, match="not SPD"):
         L.call("tg_dense_spd_inverse", L.ptr(A), 2, stream())
-    for n, bad in ((400, 399), (200, 100), (130, 0)):   # blocked path: last, middle, first block
+    # one-workgroup path, then the blocked path in its last, middle and first block
+    for n, bad in ((50, 33), (400, 399), (200, 100), (130, 0)):
         B = np.eye(n)
         B[bad, bad] = -1.0
         with pytest.raises(L.TopoHipError, match=f"not SPD \\(pivot {bad + 1}\\)"):
@@ -437,7 +438,7 @@ def test_not_spd_is_reported():
         L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), 2100, stream())
 
 
-@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 135, 193, 600, 1080, 2100])
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 97, 135, 160, 161, 192, 193, 600, 1080, 2100])
 def test_dense_spd_inverse_sizes(n):
     """One-workgroup Gauss-Jordan (n <= 64), blocked Gauss-Jordan (16-wide
     pivot blocks, n <= 2048) and the library Cholesky inverse (above) invert

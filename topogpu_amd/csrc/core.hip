@@ -157,15 +157,32 @@ __global__ void __launch_bounds__(kVecBlock) k_dot64(long long n, const double* 
 // Block r: out_k[r] = sum_b partials[k * nseg * seg + r * seg + b] in block
 // order (the layout block_partials<K> writes for a grid of nseg slices of seg
 // blocks each).
+// Each thread adds its blocks b = tid, tid + 256, ... in that order; the
+// loads go out eight at a time (the partials were just written by other
+// XCDs, so every load is a trip to memory), padded with zeros past seg.
+__device__ __forceinline__ double strided_sum(const double* pk, int seg) {
+  double v = 0.0;
+  for (int b0 = threadIdx.x; b0 < seg; b0 += 8 * blockDim.x) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * (int)blockDim.x;
+      x[u] = b < seg ? pk[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += x[u];
+  }
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_finalize(int K, int seg, const double* partials, double* out0, double* out1) {
   const int r = blockIdx.x, nseg = gridDim.x;
-  for (int k = 0; k < K; ++k) {
-    double v[1] = {0.0};
-    const double* pk = partials + (size_t)k * nseg * seg + (size_t)r * seg;
-    for (int b = threadIdx.x; b < seg; b += blockDim.x) v[0] += pk[b];
-    block_sum<1>(v);
-    double* o = k == 0 ? out0 : out1;
-    if (threadIdx.x == 0 && o) o[r] = v[0];
+  const double* p0 = partials + (size_t)r * seg;
+  double v[2] = {strided_sum(p0, seg), K > 1 ? strided_sum(p0 + (size_t)nseg * seg, seg) : 0.0};
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    if (out0) out0[r] = v[0];
+    if (K > 1 && out1) out1[r] = v[1];
   }
 }
 

@@ -205,7 +205,9 @@ __global__ void __launch_bounds__(256) k_galerkin(Grid f, Grid c, const uint8_t*
   }
 }
 
-// Row-wise dense assembly: thread per DOF of the grid.
+// Row-wise dense assembly: thread per DOF of the grid. The row's (at most
+// 27 x 3) nonzeros are summed in registers over the elements in corner order
+// and stored once each (A is zero beforehand).
 __global__ void k_assemble_dense(Grid g, const double* __restrict__ mats, const int* __restrict__ fidx, int nf,
                                  double* __restrict__ A) {
   const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -215,17 +217,32 @@ __global__ void k_assemble_dense(Grid g, const double* __restrict__ mats, const 
   const long long n = d / 3;
   const int r = (int)(d % 3);
   const int k = (int)(n % g.NZ()), j = (int)((n / g.NZ()) % g.NY()), i = (int)(n / ((long long)g.NZ() * g.NY()));
-  double* Ar = A + (long long)row * nf;
-  for (int o = 0; o < 8; ++o) {
-    const int ei = i - corner_x(o), ej = j - corner_y(o), ek = k - corner_z(o);
+  double acc[27][3];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {   // the node is corner a of element (i,j,k) - corner(a)
+    const int ei = i - corner_x(a), ej = j - corner_y(a), ek = k - corner_z(a);
     if (ei < 0 || ei >= g.nx || ej < 0 || ej >= g.ny || ek < 0 || ek >= g.nz) continue;
-    const double* K = mats + 576 * g.elem(ei, ej, ek) + (3 * o + r) * 24;
-    for (int cc = 0; cc < 8; ++cc) {
-      const long long m = g.node(ei + corner_x(cc), ej + corner_y(cc), ek + corner_z(cc));
-      for (int q = 0; q < 3; ++q) {
-        const int col = fidx[3 * m + q];
-        if (col >= 0) Ar[col] += K[3 * cc + q];
-      }
+    const double* K = mats + 576 * g.elem(ei, ej, ek) + (3 * a + r) * 24;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int q = (corner_x(b) - corner_x(a) + 1) * 9 + (corner_y(b) - corner_y(a) + 1) * 3 +
+                    (corner_z(b) - corner_z(a) + 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[q][c] += K[3 * b + c];
+    }
+  }
+  double* Ar = A + (long long)row * nf;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int a = i + q / 9 - 1, b = j + (q / 3) % 3 - 1, e = k + q % 3 - 1;
+    if (a < 0 || a >= g.NX() || b < 0 || b >= g.NY() || e < 0 || e >= g.NZ()) continue;
+    const long long m = g.node(a, b, e);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int col = fidx[3 * m + c];
+      if (col >= 0) Ar[col] = acc[q][c];
     }
   }
 }
@@ -633,17 +650,24 @@ int dense_inverse_cholesky(double* A, int n, hipStream_t st);
 // pass over the whole matrix) cost more than rocSOLVER's Cholesky
 // factorisation and inverse (potrf + potri, 45 ms at n = 7,260 on MI355X).
 constexpr int kBlockedMaxN = 2048;
+// Up to this size one workgroup holds the whole matrix in registers (NB x NB
+// FP64 entries per thread, NB = ceil(n / 32)). Each pivot step costs every
+// thread O(NB^2) selects and FMAs whatever the step, so beyond NB = 2 the
+// blocked form is faster: at n = 135, 286 us for NB = 5 against 255 us.
+constexpr int kRegisterMaxN = 64;
 
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
   TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device");
   const hipStream_t st = (hipStream_t)s;
   int* info_d = workspace().flags;
-  if (n <= 64) {
-    if (n <= 32)
-      hipLaunchKernelGGL((k_spd_inverse_gj<1>), dim3(1), dim3(GJ_THREADS), 0, st, A, n, n, A, n, info_d, 0, 1);
-    else
-      hipLaunchKernelGGL((k_spd_inverse_gj<2>), dim3(1), dim3(GJ_THREADS), 0, st, A, n, n, A, n, info_d, 0, 1);
+  if (n <= kRegisterMaxN) {
+    // one workgroup, the matrix in registers: NB x NB entries per thread
+    const dim3 g1(1), b1(GJ_THREADS);
+    switch ((n + 31) / 32) {
+      case 1: hipLaunchKernelGGL((k_spd_inverse_gj<1>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1); break;
+      default: hipLaunchKernelGGL((k_spd_inverse_gj<2>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1); break;
+    }
     TG_LAUNCH_CHECK();
   } else if (n <= kBlockedMaxN) {
     return dense_inverse_blocked(A, n, st);

@@ -494,8 +494,21 @@ __device__ void sum_candidate_partials(const double* partials, int nb, int nc, d
   for (int c0 = 0; c0 < nc; c0 += 16) {
     const int c = c0 + c16;
     double v = 0.0;
-    if (c < nc)
-      for (int b = l; b < nb; b += 16) v += partials[(size_t)c * nb + b];
+    if (c < nc) {
+      // the same additions in the same order, with eight loads in flight at
+      // a time (the partials come from other XCDs' writes: each load is a
+      // trip to memory)
+      const double* pc = partials + (size_t)c * nb;
+      int b = l;
+      for (; b + 16 * 7 < nb; b += 16 * 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = pc[b + 16 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += x[u];
+      }
+      for (; b < nb; b += 16) v += pc[b];
+    }
 #pragma unroll
     for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
     if (c < nc && l == 0) out[c] = v;
@@ -507,22 +520,27 @@ __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double vol
                                                 Workspace ws) {
   if (state[S_DONE] != 0.0) return;
   __shared__ double svol[OC_NPTS];
-  sum_candidate_partials(ws.partials, nb, OC_NPTS, svol);
-  if (threadIdx.x != 0) return;
+  __shared__ double scand[OC_NPTS];
+  if (threadIdx.x < OC_NPTS) scand[threadIdx.x] = state[S_CAND + threadIdx.x];
+  sum_candidate_partials(ws.partials, nb, OC_NPTS, svol);   // ends with a barrier
+  // one wavefront walks the bisection: lane t < OC_NPTS holds candidate t in
+  // registers, so the per-step lookup is a ballot rather than a serial scan
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
   const double dn = (double)n;
   constexpr double kMargin = 2e-14;   // on volume/n; a faithfully rounded projection moves it by ~1e-16
-  double cand[OC_NPTS];
-  for (int t = 0; t < OC_NPTS; ++t) cand[t] = state[S_CAND + t];
+  const double my_cand = lane < OC_NPTS ? scand[lane] : -1.0;
+  const int my_dec = lane < OC_NPTS ? (svol[lane] / dn > volfrac ? 1 : 0) : 0;
   double a = state[S_A], va = state[S_VA], b = state[S_B], vb = state[S_VB];
   double sa = state[S_SA], sb = state[S_SB];
   for (int t = 0; t < OC_NPTS; ++t) {
-    const double f = svol[t] / dn;
+    const double f = svol[t] / dn, ct = scand[t];
     if (f > volfrac) {
-      if (cand[t] > a) { a = cand[t]; va = svol[t]; }
-      if (f - volfrac > kMargin && cand[t] > sa) sa = cand[t];
+      if (ct > a) { a = ct; va = svol[t]; }
+      if (f - volfrac > kMargin && ct > sa) sa = ct;
     } else {
-      if (cand[t] < b) { b = cand[t]; vb = svol[t]; }
-      if (volfrac - f > kMargin && cand[t] < sb) sb = cand[t];
+      if (ct < b) { b = ct; vb = svol[t]; }
+      if (volfrac - f > kMargin && ct < sb) sb = ct;
     }
   }
   double l1 = state[S_L1], l2 = state[S_L2], lm_last = state[S_LM], steps = state[S_STEPS];
@@ -530,8 +548,9 @@ __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double vol
   while (!done) {
     const double lm = 0.5 * (l1 + l2);
     int dec = -1;   // 1: volume above target (l1 = lm), 0: at or below (l2 = lm)
-    for (int t = 0; t < OC_NPTS && dec < 0; ++t)
-      if (cand[t] == lm) dec = svol[t] / dn > volfrac ? 1 : 0;
+    // the lowest-numbered candidate equal to the midpoint decides
+    const unsigned long long hit = __ballot(lane < OC_NPTS && my_cand == lm);
+    if (hit) dec = __shfl(my_dec, __ffsll((long long)hit) - 1, 64);
     if (dec < 0 && sa >= 0.0 && lm <= sa) dec = 1;
     if (dec < 0 && sb < HUGE_VAL && lm >= sb) dec = 0;
     if (dec < 0) break;
@@ -540,9 +559,12 @@ __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double vol
     steps += 1.0;
     if (l2 - l1 < 1e-9 || steps >= max_steps) done = true;
   }
-  state[S_L1] = l1; state[S_L2] = l2; state[S_LM] = lm_last; state[S_STEPS] = steps;
-  state[S_A] = a; state[S_VA] = va; state[S_B] = b; state[S_VB] = vb;
-  state[S_SA] = sa; state[S_SB] = sb;
+  if (lane == 0) {
+    state[S_L1] = l1; state[S_L2] = l2; state[S_LM] = lm_last; state[S_STEPS] = steps;
+    state[S_A] = a; state[S_VA] = va; state[S_B] = b; state[S_VB] = vb;
+    state[S_SA] = sa; state[S_SB] = sb;
+  }
+  if (lane != 0) return;
   state[S_PASSES] += 1.0;
   state[S_DONE] = done ? 1.0 : 0.0;
   if (done) return;
