@@ -410,8 +410,19 @@ __device__ __forceinline__ void pencils0(const float4* su, float* slot, const Pe
 }
 
 // ---------------------------------------------------------------- level 0
+// Minimum waves per SIMD asked of the compiler for the PCG product (hi/lo
+// split, DOT epilogue), which then keeps it within 512 / 6 = 80 VGPRs
+// without spilling: six waves are three 8-wave workgroups per CU, so the
+// 1,518 tiles of 160x80x80 run in two rounds instead of three (24.6 against
+// 26.4 us). The single-pass instantiations fit 80 VGPRs by themselves; the
+// other split ones would spill.
 template <bool SPLIT, int MODE>
-__global__ void __launch_bounds__(Tile0::NTHR) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
+constexpr int hex_min_waves() {
+  return (SPLIT && MODE == TG_EP_DOT) ? 6 : 1;
+}
+
+template <bool SPLIT, int MODE>
+__global__ void __launch_bounds__(Tile0::NTHR, (hex_min_waves<SPLIT, MODE>())) k_hex_mfma(Grid g, TileShape ts, const uint8_t* __restrict__ mask,
                                                          const float* __restrict__ E, const float* __restrict__ u,
                                                          Ep32 ep, Workspace ws) {
   using T = Tile0;
