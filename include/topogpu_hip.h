@@ -74,7 +74,11 @@ enum {
   /* as TG_OP_Q8_F32 with the eight child products on BF16 MFMA (FP32
    * accumulate), single pass / hi-lo split operands                        */
   TG_OP_Q8_BF16 = 7,
-  TG_OP_Q8_BF16X3 = 8
+  TG_OP_Q8_BF16X3 = 8,
+  /* as TG_OP_STENCIL_F32 with the coefficients stored as BF16 (from
+   * tg_assemble_stencil_bf16: 243 x 2 bytes per node, half the bytes the
+   * bandwidth-bound coarse levels read); products and sums in FP32        */
+  TG_OP_STENCIL_BF16 = 9
 };
 
 typedef struct {
@@ -83,7 +87,8 @@ typedef struct {
   const uint8_t* mask;        /* per-node free mask of that grid             */
   const float* weights;       /* HEX*: E[elem]; Q8: fine E[fine elem]        */
   const float* elem_mats;     /* STORED: 576 per element                     */
-  const float* stencil;       /* STENCIL: coefficients, see tg_assemble_stencil */
+  const void* stencil;        /* STENCIL_F32: float, STENCIL_BF16: BF16 bits;
+                                 see tg_assemble_stencil                     */
 } tg_op32;
 
 enum {
@@ -131,6 +136,11 @@ typedef struct {
  * K_l = R K_{l-1} P of multigrid_v4.py:747-752 on the V-cycle path. */
 int tg_assemble_stencil(const tg_grid* g, const uint8_t* mask, const double* mats, float* coef,
                         void* stream);
+/* The same coefficients rounded to BF16 (round to nearest even of the FP32
+ * value tg_assemble_stencil stores), as raw 16-bit patterns in the same
+ * layout, for TG_OP_STENCIL_BF16. */
+int tg_assemble_stencil_bf16(const tg_grid* g, const uint8_t* mask, const double* mats, uint16_t* coef,
+                             void* stream);
 
 /* y (or the epilogue outputs) = A u for a single-precision operator.
  * Replaces FusedMatvec.matvec / _launch (cuda_fused_matvec.py:336-409),

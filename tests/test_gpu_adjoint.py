@@ -47,7 +47,11 @@ def _c3():
 @pytest.fixture(scope="module")
 def c3(gpu):
     free, F, ndof, rho, elems = _c3()
-    comp = HipCompliance(DIMS, free, F, rmin=1.5, precision="bf16", tol=1e-11)
+    # the adjoint gradient errs to first order in the state's solve error
+    # (its FD check does not): at tol 1e-12 the largest resolvable relative
+    # error is ~2e-7, at 1e-11 it reaches ~1e-6 depending on the
+    # preconditioner's rounding
+    comp = HipCompliance(DIMS, free, F, rmin=1.5, precision="bf16", tol=1e-12)
     r = torch.tensor(rho, dtype=torch.float64, device=DEV, requires_grad=True)
     C = comp(r)
     C.backward()

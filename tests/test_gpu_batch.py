@@ -67,13 +67,17 @@ def _ops(dims, rng):
            L.ptr(m32), stream())
     coef = torch.zeros(243 * nc, **F32)
     L.call("tg_assemble_stencil", L.grid(dc), L.ptr(dm1), L.ptr(m64), L.ptr(coef), stream())
-    keep += [dm1, m64, m32, coef]
+    coef16 = torch.zeros(243 * nc, dtype=torch.int16, device=DEV)
+    L.call("tg_assemble_stencil_bf16", L.grid(dc), L.ptr(dm1), L.ptr(m64), L.ptr(coef16), stream())
+    keep += [dm1, m64, m32, coef, coef16]
     free1 = _free(m1)
     for kind in (L.TG_OP_Q8_F32, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16X3):
         out.append((f"q8_{kind}", L.TgOp32(kind, L.grid(dc), L.ptr(dm1), L.ptr(dE32), None), 3 * nc, free1))
     out.append(("stored", L.TgOp32(L.TG_OP_STORED_F32, L.grid(dc), L.ptr(dm1), None, L.ptr(m32)), 3 * nc, free1))
     out.append(("stencil", L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dc), L.ptr(dm1), None, None, L.ptr(coef)), 3 * nc,
                 free1))
+    out.append(("stencil_bf16", L.TgOp32(L.TG_OP_STENCIL_BF16, L.grid(dc), L.ptr(dm1), None, None, L.ptr(coef16)),
+                3 * nc, free1))
     return out, keep
 
 

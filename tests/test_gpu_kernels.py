@@ -287,6 +287,23 @@ def _stencil_op(dims, mask, m64):
     return L.TgOp32(L.TG_OP_STENCIL_F32, L.grid(dims), L.ptr(dm), None, None, L.ptr(coef)), coef, dm
 
 
+def _stencil_apply_host(dims, coef, u):
+    """y = A u in FP64 from stencil coefficients coef[(c*81 + 3*q + d) * nodes
+    + n] (neighbour q = (dx+1)*9 + (dy+1)*3 + dz+1, node n = (i*NY + j)*NZ + k)."""
+    NX, NY, NZ = (d + 1 for d in dims)
+    nn = NX * NY * NZ
+    cf = coef.reshape(3, 27, 3, nn)
+    up = np.pad(u.reshape(NX, NY, NZ, 3), ((1, 1), (1, 1), (1, 1), (0, 0)))
+    y = np.zeros((nn, 3))
+    for q in range(27):
+        dx, dy, dz = q // 9 - 1, (q // 3) % 3 - 1, q % 3 - 1
+        un = up[1 + dx:1 + dx + NX, 1 + dy:1 + dy + NY, 1 + dz:1 + dz + NZ].reshape(nn, 3)
+        for c in range(3):
+            for d in range(3):
+                y[:, c] += cf[c, q, d] * un[:, d]
+    return y.ravel()
+
+
 def _check_stencil_epilogues(op, K, fr, nd, rng, tol=2e-6):
     """Every epilogue of a coarse-level operator against the sparse Galerkin
     matrix K (on the free DOFs fr of an nd-vector)."""
@@ -355,6 +372,20 @@ def test_galerkin_levels_match_ptkp(dims, point):
         op_s, coef, dm_s = _stencil_op(dc, masks[lv], m64)
         assert not torch.isnan(coef).any()
         _check_stencil_epilogues(op_s, Kc, fr, nd, rng)
+        # BF16 storage: the bits are the FP32 coefficients rounded to nearest
+        # even, and the operator applies exactly that rounded stencil
+        cb = torch.zeros(243 * (nd // 3), dtype=torch.int16, device=DEV)
+        L.call("tg_assemble_stencil_bf16", L.grid(dc), L.ptr(dm_s), L.ptr(m64), L.ptr(cb), stream())
+        bits = h(coef).view(np.uint32).astype(np.uint64)
+        rne = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16).astype(np.uint16)
+        assert np.array_equal(h(cb).view(np.uint16), rne)
+        op_b = L.TgOp32(L.TG_OP_STENCIL_BF16, L.grid(dc), L.ptr(dm_s), None, None, L.ptr(cb))
+        yb = torch.zeros(nd, dtype=torch.float32, device=DEV)
+        u32 = u.astype(np.float32)
+        L.call("tg_apply32", op_b, L.ptr(t(u32)), L.ptr(yb), L.ep(L.TG_EP_STORE), stream())
+        cref = (rne.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        assert rel_l2(h(yb), _stencil_apply_host(dc, cref, u32.astype(np.float64))) < 1e-6
+        _check_stencil_epilogues(op_b, Kc, fr, nd, rng, tol=1e-2)
         assert np.array_equal(fem.free_from_node_mask(masks[lv]), fr)
         if lv == 1 and not point and all(d % 2 == 0 for d in dims):
             yq = torch.zeros(nd, dtype=torch.float32, device=DEV)

@@ -21,7 +21,8 @@ namespace tg {
 int launch_hex_mfma(const Grid& g, const uint8_t* mask, const float* E, const float* u, const Ep32& ep,
                     double* dot, bool split, int nrhs, hipStream_t s);
 int launch_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s);
-int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
+template <typename C>
+int launch_stencil(const Grid& g, const uint8_t* mask, const C* coef, const float* u, const Ep32& ep,
                    double* dot, int nrhs, hipStream_t s);
 int launch_q8_mfma(const Grid& g, const uint8_t* mask, const float* Ef, const float* u, const Ep32& ep, double* dot,
                    bool split, int nrhs, hipStream_t s);
@@ -412,7 +413,10 @@ extern "C" int tg_apply32_b(const tg_op32* op, int32_t nrhs, const float* u, flo
       return launch_q8_mfma(g, op->mask, op->weights, u, e, ep->dot, op->kind == TG_OP_Q8_BF16X3, nrhs, s);
     case TG_OP_STENCIL_F32:
       TG_REQUIRE(op->stencil, "stencil NULL");
-      return launch_stencil(g, op->mask, op->stencil, u, e, ep->dot, nrhs, s);
+      return launch_stencil(g, op->mask, static_cast<const float*>(op->stencil), u, e, ep->dot, nrhs, s);
+    case TG_OP_STENCIL_BF16:
+      TG_REQUIRE(op->stencil, "stencil NULL");
+      return launch_stencil(g, op->mask, static_cast<const uint16_t*>(op->stencil), u, e, ep->dot, nrhs, s);
     default:
       return set_error(TG_ERR_ARG, "unknown operator kind %d", op->kind);
   }

@@ -23,10 +23,24 @@ namespace {
 
 constexpr int SNT = 256;
 
+// Coefficient storage: FP32, or BF16 bits (round to nearest even of the FP32
+// value; a BF16 value widens to FP32 exactly).
+__device__ __forceinline__ void coef_store(float* p, float v) { *p = v; }
+__device__ __forceinline__ void coef_store(uint16_t* p, float v) {
+  uint32_t u = __builtin_bit_cast(uint32_t, v);
+  u += 0x7fffu + ((u >> 16) & 1u);   // finite values only (sums of finite element entries)
+  *p = (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float coef_load(const float* p) { return *p; }
+__device__ __forceinline__ float coef_load(const uint16_t* p) {
+  return __builtin_bit_cast(float, (uint32_t)*p << 16);
+}
+
 // One thread per (node, row component): sum over the elements containing the
 // node of their 3 x 24 row block, in FP64, then masked and rounded to FP32.
+template <typename C>
 __global__ void __launch_bounds__(SNT) k_assemble_stencil(Grid g, const uint8_t* __restrict__ mask,
-                                                         const double* __restrict__ mats, float* __restrict__ coef) {
+                                                         const double* __restrict__ mats, C* __restrict__ coef) {
   const long long n = blockIdx.x * (long long)SNT + threadIdx.x;
   const int c = blockIdx.y;
   const long long nn = g.nodes();
@@ -59,7 +73,7 @@ __global__ void __launch_bounds__(SNT) k_assemble_stencil(Grid g, const uint8_t*
     const unsigned mcol = in ? mask[g.node(a, b, e)] : 0u;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      coef[(long long)(c * 81 + 3 * q + d) * nn + n] = (mcol >> d & 1u) ? (float)acc[q][d] : 0.0f;
+      coef_store(&coef[(long long)(c * 81 + 3 * q + d) * nn + n], (mcol >> d & 1u) ? (float)acc[q][d] : 0.0f);
   }
 }
 
@@ -71,9 +85,9 @@ __global__ void __launch_bounds__(SNT) k_assemble_stencil(Grid g, const uint8_t*
 // epilogue.
 constexpr int SAP = 64;
 
-template <int MODE>
+template <typename C, int MODE>
 __global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t* __restrict__ mask,
-                                                          const float* __restrict__ coef, const float* __restrict__ u,
+                                                          const C* __restrict__ coef, const float* __restrict__ u,
                                                           Ep32 ep_all, Workspace ws) {
   using Mode = EpMode<MODE>;
   __shared__ float plane_sum[2][SAP];
@@ -99,7 +113,7 @@ __global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t
     uo = in_at(3 * nc + c);
   }
   const long long sz = g.NZ(), sy = (long long)g.NZ() * g.NY();
-  const float* cf = coef + (long long)(c * 81 + 27 * w) * nn + nc;
+  const C* cf = coef + (long long)(c * 81 + 27 * w) * nn + nc;
   const long long base = nc + (w - 1) * sy;
   float au = 0.0f;
 #pragma unroll
@@ -107,7 +121,7 @@ __global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t
     long long nb = base + (j / 3 - 1) * sz + (j % 3 - 1);
     nb = nb < 0 ? 0 : (nb >= nn ? nn - 1 : nb);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) au = fmaf(cf[(long long)(3 * j + d) * nn], in_at(3 * nb + d), au);
+    for (int d = 0; d < 3; ++d) au = fmaf(coef_load(cf + (long long)(3 * j + d) * nn), in_at(3 * nb + d), au);
   }
   if (w < 2) plane_sum[w][lane] = au;
   __syncthreads();
@@ -125,18 +139,36 @@ __global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t
 
 }  // namespace
 
-int launch_stencil(const Grid& g, const uint8_t* mask, const float* coef, const float* u, const Ep32& ep,
+template <typename C>
+int launch_stencil(const Grid& g, const uint8_t* mask, const C* coef, const float* u, const Ep32& ep,
                    double* dot, int nrhs, hipStream_t s) {
   const long long nn = g.nodes();
   const long long nbx = (nn + SAP - 1) / SAP;
   TG_REQUIRE(nbx * 3 * nrhs <= kMaxPartials, "grid too large for the reduction workspace");
   return with_mode(ep, [&](const auto& em) {
     constexpr int M = std::decay_t<decltype(em)>::kMode;
-    hipLaunchKernelGGL((k_stencil_apply<M>), dim3((unsigned)nbx, 3, nrhs), dim3(3 * SAP), 0, s, g, mask, coef, u,
+    hipLaunchKernelGGL((k_stencil_apply<C, M>), dim3((unsigned)nbx, 3, nrhs), dim3(3 * SAP), 0, s, g, mask, coef, u,
                        ep, workspace());
     TG_LAUNCH_CHECK();
     return EpMode<M>::kReduces ? finalize_batched(1, (int)(nbx * 3), nrhs, dot, nullptr, s) : TG_OK;
   });
+}
+
+template int launch_stencil<float>(const Grid&, const uint8_t*, const float*, const float*, const Ep32&, double*,
+                                   int, hipStream_t);
+template int launch_stencil<uint16_t>(const Grid&, const uint8_t*, const uint16_t*, const float*, const Ep32&,
+                                      double*, int, hipStream_t);
+
+template <typename C>
+int assemble_stencil(const tg_grid* gp, const uint8_t* mask, const double* mats, C* coef, void* stream) {
+  if (int rc = check_grid(gp)) return rc;
+  TG_REQUIRE(mask && mats && coef, "assemble_stencil: NULL argument");
+  const Grid g = to_grid(gp);
+  const long long nn = g.nodes();
+  hipLaunchKernelGGL((k_assemble_stencil<C>), dim3((unsigned)((nn + SNT - 1) / SNT), 3), dim3(SNT), 0,
+                     (hipStream_t)stream, g, mask, mats, coef);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
 }
 
 }  // namespace tg
@@ -145,12 +177,9 @@ using namespace tg;
 
 extern "C" int tg_assemble_stencil(const tg_grid* gp, const uint8_t* mask, const double* mats, float* coef,
                                    void* stream) {
-  if (int rc = check_grid(gp)) return rc;
-  TG_REQUIRE(mask && mats && coef, "assemble_stencil: NULL argument");
-  const Grid g = to_grid(gp);
-  const long long nn = g.nodes();
-  hipLaunchKernelGGL(k_assemble_stencil, dim3((unsigned)((nn + SNT - 1) / SNT), 3), dim3(SNT), 0,
-                     (hipStream_t)stream, g, mask, mats, coef);
-  TG_LAUNCH_CHECK();
-  return TG_OK;
+  return assemble_stencil(gp, mask, mats, coef, stream);
+}
+extern "C" int tg_assemble_stencil_bf16(const tg_grid* gp, const uint8_t* mask, const double* mats, uint16_t* coef,
+                                        void* stream) {
+  return assemble_stencil(gp, mask, mats, coef, stream);
 }

@@ -10,7 +10,10 @@ Restates the hierarchy of ``GalerkinMatFreeGMG`` (src/gpu_fem/multigrid_v4.py:
   fine E (no stored matrices on the V-cycle path), otherwise stored;
 * levels >= 2 are Galerkin element matrices built on the device from the
   level above (replacing the SpGEMM R @ (K @ P), multigrid_v4.py:747-752),
-  assembled into 27-point block stencils for the V-cycle;
+  assembled into 27-point block stencils for the V-cycle (FP32
+  coefficients, or BF16 ones: these levels are bound by reading them, and
+  BF16 coefficients leave the PCG iteration count within 1 % on the
+  160x80x80 workload);
 * the coarsest level is solved with an explicit FP64 dense inverse computed
   once per setup (Gauss-Jordan elimination; it replaces the dense Cholesky
   factorisation of multigrid_v4.py:774-811).
@@ -24,8 +27,8 @@ the step rather than to the whole (smooth, large) iterate, and the
 pre-smoother hands its residual to the restriction without another product.
 Each step is one operator launch with the update fused into its epilogue.
 The one smooth product of the cycle, A * (P e_c) after the coarse
-correction, uses the accurate operator of the level. Coarse levels run in
-FP32.
+correction, uses the accurate operator of the level. Coarse levels compute
+in FP32.
 """
 
 from __future__ import annotations
@@ -43,7 +46,7 @@ class _Level:
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
-                L.TG_OP_STORED_F32: "stored_f32", L.TG_OP_STENCIL_F32: "stencil_f32",
+                L.TG_OP_STORED_F32: "stored_f32", L.TG_OP_STENCIL_F32: "stencil_f32", L.TG_OP_STENCIL_BF16: "stencil_bf16",
                 L.TG_OP_Q8_BF16: "galerkin_q8_bf16_mfma", L.TG_OP_Q8_BF16X3: "galerkin_q8_bf16x3_mfma"}
 
 _EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
@@ -74,9 +77,11 @@ class GMG:
                  l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree=2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
-                 nrhs: int = 1):
+                 nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
+        if coarse_stencil not in (L.TG_OP_STENCIL_F32, L.TG_OP_STENCIL_BF16):
+            raise ValueError("coarse_stencil must be TG_OP_STENCIL_F32 or TG_OP_STENCIL_BF16")
         self.dev = device
         self.nrhs = int(nrhs)
         if not 1 <= self.nrhs <= 64:
@@ -126,7 +131,7 @@ class GMG:
         for lv in range(1, n_levels):
             f, c = self.levels[lv - 1], self.levels[lv]
             c.consistent = fem.masks_consistent(f.dims, c.dims, f.mask_host, c.mask_host)
-            c.kind = L.TG_OP_STENCIL_F32
+            c.kind = coarse_stencil
         l1 = self.levels[1]
         if level1 not in ("q8", "stencil"):
             raise ValueError(f"level1 must be 'q8' or 'stencil'; got {level1!r}")
@@ -169,7 +174,7 @@ class GMG:
     def op(self, lv: int, accurate: bool = False) -> L.TgOp32:
         Lv = self.levels[lv]
         kind = Lv.kind_accurate if accurate else Lv.kind
-        if kind == L.TG_OP_STENCIL_F32:
+        if kind in (L.TG_OP_STENCIL_F32, L.TG_OP_STENCIL_BF16):
             return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), None, None, L.ptr(Lv.stencil))
         return L.TgOp32(kind, Lv.grid, L.ptr(Lv.mask), L.ptr(self.E32), None, None)
 
@@ -210,9 +215,12 @@ class GMG:
                 if lv == 1 and self.l1_q8:
                     L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
                 else:
+                    bf16 = c.kind == L.TG_OP_STENCIL_BF16
                     if c.stencil is None:
-                        c.stencil = torch.empty(243 * c.nodes, dtype=torch.float32, device=self.dev)
-                    L.call("tg_assemble_stencil", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.stencil), stream)
+                        c.stencil = torch.empty(243 * c.nodes, dtype=torch.int16 if bf16 else torch.float32,
+                                                device=self.dev)
+                    L.call("tg_assemble_stencil_bf16" if bf16 else "tg_assemble_stencil", c.grid, L.ptr(c.mask),
+                           L.ptr(c.mats64), L.ptr(c.stencil), stream)
                     L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
         last = self.levels[-1]
         L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,

@@ -14,7 +14,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent.parent / "libtopohip.so"
 
 TG_OP_HEX_F32, TG_OP_HEX_BF16, TG_OP_HEX_BF16X3, TG_OP_Q8_F32, TG_OP_STORED_F32, TG_OP_STENCIL_F32 = 1, 2, 3, 4, 5, 6
-TG_OP_Q8_BF16, TG_OP_Q8_BF16X3 = 7, 8
+TG_OP_Q8_BF16, TG_OP_Q8_BF16X3, TG_OP_STENCIL_BF16 = 7, 8, 9
 (TG_EP_STORE, TG_EP_DOT, TG_EP_RESID, TG_EP_RESID_NORM, TG_EP_CHEB, TG_EP_CHEB_R, TG_EP_RESID_UPDATE,
  TG_EP_CHEB_FIRST, TG_EP_RESID_FIRST, TG_EP_CHEB_LAST) = range(10)
 OC_STATE_LEN = 32   # device doubles of the OC bisection state (TG_OC_STATE_LEN)
@@ -66,6 +66,7 @@ _SIGS = {
     "tg_galerkin_from_fine": [C.POINTER(TgGrid), C.c_int, _vp, _vp, _vp, _vp, _vp],
     "tg_assemble_dense": [C.POINTER(TgGrid), _vp, _vp, _i32, _vp, _vp],
     "tg_assemble_stencil": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
+    "tg_assemble_stencil_bf16": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_dense_spd_inverse": [_vp, _i32, _vp],
     "tg_coarsest_apply": [_vp, _i32, _vp, _i64, _vp, _vp, _vp],
     "tg_dot32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -34,11 +34,12 @@ from .gmg import GMG, launch_op
 
 # precision -> (Krylov operator, level-0 smoother operator, level-0
 #               post-correction operator, level-1 post-correction operator,
-#               level-1 smoother operator)
+#               level-1 smoother operator, coarse-level stencil storage)
 #   bf16       : split BF16 MFMA (hi/lo operands, 3 passes) for the PCG
 #                product, single-pass BF16 MFMA in the whole V-cycle (split
 #                post-corrections gave the same iteration counts on the
-#                160x80x80 workload at 2% more time per iteration)
+#                160x80x80 workload at 2% more time per iteration), BF16
+#                coarse-level stencil coefficients
 #   bf16-plain : single-pass BF16 MFMA everywhere (the inner solve then
 #                stalls near the BF16 rounding level; kept for study)
 #   bf16x3     : split BF16 MFMA everywhere
@@ -48,10 +49,14 @@ from .gmg import GMG, launch_op
 # correction c) and the power iteration for lambda_max use the
 # post-correction operator of their level.
 PRECISIONS = {
-    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
-    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16),
-    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16X3),
-    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_Q8_F32, L.TG_OP_Q8_F32),
+    "bf16": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16,
+             L.TG_OP_STENCIL_BF16),
+    "bf16-plain": (L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_HEX_BF16, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16,
+                   L.TG_OP_STENCIL_BF16),
+    "bf16x3": (L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_HEX_BF16X3, L.TG_OP_Q8_BF16X3, L.TG_OP_Q8_BF16X3,
+               L.TG_OP_STENCIL_F32),
+    "fp32": (L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_HEX_F32, L.TG_OP_Q8_F32, L.TG_OP_Q8_F32,
+             L.TG_OP_STENCIL_F32),
 }
 
 
@@ -112,7 +117,7 @@ class HipStateSolver:
         self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[2],
                        l1_q8_op=ops[4], l1_q8_accurate_op=ops[3], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
-                       level1=level1, nrhs=self.nrhs, device=self.dev)
+                       level1=level1, nrhs=self.nrhs, coarse_stencil=ops[5], device=self.dev)
         nb = self.nrhs * self.n
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.u = torch.zeros(nb, dtype=torch.float64, device=self.dev)
