@@ -62,7 +62,8 @@ def _log(msg: str) -> None:
 # measured with hardware counters (FETCH_SIZE and WRITE_SIZE runs of
 # ``bench.py --kernel-probe``, scaled by the counts of a 256 MiB copy); the
 # summary file holds the commands and the figures.
-TRAFFIC_PROFILES = {"hex_bf16_mfma/CHEB_R": "profiles/hex_mfma_cheb_r_160x80x80.md"}
+TRAFFIC_PROFILES = {"hex_bf16_mfma/CHEB_R": "profiles/hex_mfma_cheb_r_160x80x80.md",
+                    "hex_bf16x3_mfma/DOT": "profiles/hex_mfma_x3_dot_160x80x80.md"}
 
 
 def profiled_traffic(kernel: str):
@@ -210,8 +211,10 @@ def saturated_kernel_times(eng, counter, reps: int = 50) -> dict:
 
 
 def kernel_probe(precision: str, reps: int = 20) -> dict:
-    """Reference copy of known size, then the dominant level-0 kernel on live
-    operands: the pair a counter run scales its byte counts with."""
+    """Reference copy of known size, then the two heaviest level-0 kernels on
+    live operands (the smoother's CHEB_R step and the PCG product, within a
+    few percent of each other): what a counter run scales its byte counts
+    with."""
     import torch
 
     from topogpu_amd.hip import lib as L
@@ -233,10 +236,15 @@ def kernel_probe(precision: str, reps: int = 20) -> dict:
     ep = L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1)
     for _ in range(reps):
         L.call("tg_apply32", op, L.ptr(u), None, ep, st)
+    fine = eng.fea
+    y = torch.zeros_like(lv0.da)
+    dot = torch.zeros(1, dtype=torch.float64, device=y.device)
+    for _ in range(reps):
+        L.call("tg_apply32", fine.fine_op(), L.ptr(u), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), st)
     torch.cuda.synchronize()
-    return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "kernel": "k_hex_mfma<false, CHEB_R>",
-            "launches": reps,
-            "algorithmic_bytes_per_launch": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_R)}
+    return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "launches": reps,
+            "kernels": {"k_hex_mfma<false, CHEB_R>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_R),
+                        "k_hex_mfma<true, DOT>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_DOT)}}
 
 
 def matvec_bench(reps: int = 200, warmup: int = 5):
@@ -366,12 +374,12 @@ def main() -> int:
     ap.add_argument("--levels", default=None, help="override the hierarchy depth of the headline run (int or auto)")
     ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
     ap.add_argument("--kernel-probe", action="store_true",
-                    help="two warm-up SIMP iterations, then a 256 MiB device copy and 20 launches of the level-0 "
-                         "BF16 CHEB_R operator on the live operands (for hardware-counter runs)")
+                    help="two warm-up SIMP iterations, then a 256 MiB device copy and 20 launches each of the "
+                         "level-0 BF16 CHEB_R smoother step and the split-BF16 DOT PCG product on the live "
+                         "operands (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this path has no multi-GPU mode (see DESIGN.md for how it would "
-              "shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this path has no multi-GPU mode; run with --gpus 1", file=sys.stderr)
         return 2
     import torch
 
