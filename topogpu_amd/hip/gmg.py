@@ -310,20 +310,32 @@ class GMG:
             self._launch(lv, op, d, None, L.ep(last(i), dinv=Lv.dinv, d=dn, x=Lv.x, r=Lv.r, a1=a1, a2=a2), stream)
             d, dn = dn, d
 
+    def _degree(self, lv: int) -> int:
+        return self.fine_degree if lv == 0 else self.coarse_degrees[min(lv, len(self.coarse_degrees)) - 1]
+
+    def _vcycle_down(self, lv: int, rhs, stream) -> None:
+        """First half of a level's cycle: pre-smoothing, then the restriction
+        of its residual to the next level's right-hand side."""
+        Lv, C = self.levels[lv], self.levels[lv + 1]
+        self._presmooth(lv, rhs, self._degree(lv), stream)
+        L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
+
+    def _vcycle_up(self, lv: int, stream):
+        """Second half: the coarser cycle, prolongation, post-smoothing."""
+        Lv, C = self.levels[lv], self.levels[lv + 1]
+        ec = self._vcycle(lv + 1, C.rhs, stream)
+        L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
+        self._postsmooth(lv, self._degree(lv), stream)
+        return Lv.x
+
     def _vcycle(self, lv: int, rhs, stream):
         Lv = self.levels[lv]
         if lv == len(self.levels) - 1:
             L.call("tg_coarsest_apply_b", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
                    self.nrhs, L.ptr(rhs), L.ptr(Lv.x), stream)
             return Lv.x
-        deg = self.fine_degree if lv == 0 else self.coarse_degrees[min(lv, len(self.coarse_degrees)) - 1]
-        self._presmooth(lv, rhs, deg, stream)
-        C = self.levels[lv + 1]
-        L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
-        ec = self._vcycle(lv + 1, C.rhs, stream)
-        L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
-        self._postsmooth(lv, deg, stream)
-        return Lv.x
+        self._vcycle_down(lv, rhs, stream)
+        return self._vcycle_up(lv, stream)
 
     def graph_key(self) -> tuple:
         """What a captured V-cycle bakes in besides the level buffers made at
@@ -340,3 +352,16 @@ class GMG:
         if not self.is_setup:
             raise RuntimeError("GMG.setup() must be called before apply()")
         return self._vcycle(0, r, stream)
+
+    # apply() in two parts, so a caller can queue the first while it waits
+    # on the host for something else: begin() writes only the hierarchy's own
+    # buffers (level-0 pre-smoothing and the restriction to level 1), so it
+    # may be issued speculatively and abandoned.
+    def begin(self, r, stream) -> None:
+        if not self.is_setup:
+            raise RuntimeError("GMG.setup() must be called before apply()")
+        self._vcycle_down(0, r, stream)
+
+    def finish(self, stream):
+        """The rest of the cycle begin() started; returns z as apply() does."""
+        return self._vcycle_up(0, stream)

@@ -83,7 +83,8 @@ class HipStateSolver:
                  fine_degree: int = 2, coarse_degree=(1, 4), cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
-                 warm_start: bool = True, level1: str = "q8", graphs: bool = False, device=None):
+                 warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
+                 device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -129,12 +130,16 @@ class HipStateSolver:
         # device scalars, one row of nrhs per quantity:
         # 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 compliance
         self.s = torch.zeros(8, self.nrhs, dtype=torch.float64, device=self.dev)
+        self._rr_host = torch.zeros(self.nrhs, dtype=torch.float64, pin_memory=True)
+        self._rr_ready = torch.cuda.Event()
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
         # one Krylov iteration as a captured launch sequence (first / later)
         self.graphs = bool(graphs)
         self._graphs: dict = {}
+        # queue the next V-cycle's first part while the host waits for r.r
+        self.speculate = bool(speculate)
 
     @property
     def stream(self):
@@ -183,12 +188,15 @@ class HipStateSolver:
             self._iteration(True, st)
             its += 1
             while True:
-                rr = s[2].cpu().numpy()
+                # the next V-cycle's first part runs while the host waits for
+                # r.r (wasted, harmlessly, when this was the last iteration)
+                speculate = self.speculate and not self.graphs
+                rr = self._read_rr(st, speculate)
                 if not np.all(np.isfinite(rr)):
                     raise FloatingPointError("non-finite inner residual")
                 if np.all(np.sqrt(np.maximum(rr, 0.0)) <= inner_target) or its >= self.max_krylov:
                     break
-                self._iteration(False, st)
+                self._iteration(False, st, begun=speculate)
                 its += 1
             L.call("tg_axpy_f32_to_f64", nr * n, L.ptr(self.u), L.ptr(self.d), st)
             rnorm = self.residual64()
@@ -198,7 +206,20 @@ class HipStateSolver:
         return self.last
 
     # ------------------------------------------------------------------
-    def _issue_iteration(self, first: bool, st) -> None:
+    def _read_rr(self, st, speculate: bool) -> np.ndarray:
+        """r.r of the last iteration on the host. With speculate, the next
+        V-cycle's first part (gmg.begin, which touches only the hierarchy's
+        buffers) is queued behind the copy before the host waits, so the GPU
+        is not idle while the host reads the value and queues the rest."""
+        if not speculate:
+            return self.s[2].cpu().numpy()
+        self._rr_host.copy_(self.s[2], non_blocking=True)
+        self._rr_ready.record()
+        self.gmg.begin(self.r, st)
+        self._rr_ready.synchronize()
+        return self._rr_host.numpy().copy()
+
+    def _issue_iteration(self, first: bool, st, begun: bool = False) -> None:
         """One flexible-PCG iteration on stream st: z = M r, then
         first: d = 0, p = z;  later: p = z + beta p (beta = -(z.q)/(p.q));
         then q = A p and x, r updated with alpha = (r.z)/(p.q), rr = r.r.
@@ -206,7 +227,7 @@ class HipStateSolver:
         n, nr, s = self.n, self.nrhs, self.s
         if first:
             L.call("tg_fill32", nr * n, L.ptr(self.d), 0.0, st)
-        z = self.gmg.apply(self.r, st)
+        z = self.gmg.finish(st) if begun else self.gmg.apply(self.r, st)
         if first:
             L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), None, L.ptr(s[0]), None, st)
             L.call("tg_copy32", nr * n, L.ptr(self.p), L.ptr(z), st)
@@ -218,14 +239,14 @@ class HipStateSolver:
         L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
                L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
 
-    def _iteration(self, first: bool, st) -> None:
+    def _iteration(self, first: bool, st, begun: bool = False) -> None:
         """Issue one iteration: replay its captured graph (captured again when
         what it baked in changed: the GMG's coefficients and buffers, the
         modulus buffer), or launch it directly when graphs are off or a
         timer records per-launch events."""
         timer = self.gmg.timer
-        if not self.graphs or (timer is not None and timer.events):
-            self._issue_iteration(first, st)
+        if begun or not self.graphs or (timer is not None and timer.events):
+            self._issue_iteration(first, st, begun)
             return
         key = (self.gmg.graph_key(), L.ptr(self.E32))
         g = self._graphs.get(first)
