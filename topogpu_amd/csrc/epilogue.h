@@ -1,5 +1,10 @@
 // Per-node output stage shared by every operator kernel.
 //
+// The update formulas are written with explicit fmaf: a multiply-add left to
+// the compiler may be contracted either way round, and differently in
+// different kernels; written out, every kernel that applies an epilogue
+// rounds it the same way.
+//
 // The operator hands over A u at one owned node (three components), the
 // node's free mask and the node's own (masked) input values. The epilogue is
 // split in two phases so that kernels can issue every global load of all
@@ -97,21 +102,21 @@ struct EpMode {
       part = (double)v * (double)v;
     } else if constexpr (MODE == TG_EP_CHEB) {
       const float t = q * (p - au);
-      const float dn = f ? ((e.a2 != 0.0f) ? e.a1 * t + e.a2 * s : e.a1 * t) : 0.0f;
+      const float dn = f ? ((e.a2 != 0.0f) ? fmaf(e.a1, t, e.a2 * s) : e.a1 * t) : 0.0f;
       a.template st<EV_D>(c, dn);
       a.template st<EV_X>(c, f ? u + dn : 0.0f);
     } else if constexpr (MODE == TG_EP_CHEB_R) {
       const float rn = p - au;
-      const float dn = e.a1 * q * rn + e.a2 * u;
+      const float dn = fmaf(e.a1 * q, rn, e.a2 * u);
       a.template st<EV_R>(c, f ? rn : 0.0f);
       a.template st<EV_D>(c, f ? dn : 0.0f);
-      a.template st<EV_X>(c, f ? s + e.a3 * u + dn : 0.0f);
+      a.template st<EV_X>(c, f ? fmaf(e.a3, u, s) + dn : 0.0f);
     } else if constexpr (MODE == TG_EP_RESID_UPDATE) {
       a.template st<EV_R>(c, f ? p - au : 0.0f);
     } else if constexpr (MODE == TG_EP_CHEB_FIRST) {
       // u is d0 here: CHEB_R's arithmetic from the start x = d0, r = rhs
       const float rn = p - au;
-      const float dn = e.a1 * q * rn + e.a2 * u;
+      const float dn = fmaf(e.a1 * q, rn, e.a2 * u);
       a.template st<EV_R>(c, f ? rn : 0.0f);
       a.template st<EV_D>(c, f ? dn : 0.0f);
       a.template st<EV_X>(c, f ? u + dn : 0.0f);
@@ -120,8 +125,8 @@ struct EpMode {
       a.template st<EV_X>(c, f ? u : 0.0f);
     } else if constexpr (MODE == TG_EP_CHEB_LAST) {
       const float rn = p - au;
-      const float dn = e.a1 * q * rn + e.a2 * u;
-      a.template st<EV_X>(c, f ? s + e.a3 * u + dn : 0.0f);
+      const float dn = fmaf(e.a1 * q, rn, e.a2 * u);
+      a.template st<EV_X>(c, f ? fmaf(e.a3, u, s) + dn : 0.0f);
     }
     return part;
   }

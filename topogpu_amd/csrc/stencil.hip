@@ -17,24 +17,12 @@
 #include "common.h"
 #include "epilogue.h"
 #include "reduce.h"
+#include "stencil_coef.h"
 
 namespace tg {
 namespace {
 
 constexpr int SNT = 256;
-
-// Coefficient storage: FP32, or BF16 bits (round to nearest even of the FP32
-// value; a BF16 value widens to FP32 exactly).
-__device__ __forceinline__ void coef_store(float* p, float v) { *p = v; }
-__device__ __forceinline__ void coef_store(uint16_t* p, float v) {
-  uint32_t u = __builtin_bit_cast(uint32_t, v);
-  u += 0x7fffu + ((u >> 16) & 1u);   // finite values only (sums of finite element entries)
-  *p = (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float coef_load(const float* p) { return *p; }
-__device__ __forceinline__ float coef_load(const uint16_t* p) {
-  return __builtin_bit_cast(float, (uint32_t)*p << 16);
-}
 
 // One thread per (node, row component): sum over the elements containing the
 // node of their 3 x 24 row block, in FP64, then masked and rounded to FP32.
@@ -103,7 +91,7 @@ __global__ void __launch_bounds__(3 * SAP) k_stencil_apply(Grid g, const uint8_t
   const long long nc = own ? n : nn - 1;
   // the operator's input (prescaled modes: the Chebyshev start a3 * dinv * u)
   auto in_at = [&](long long i) -> float {
-    if constexpr (Mode::kPrescale) return (ep.a3 * ep.dinv[i]) * u[i];
+    if constexpr (Mode::kPrescale) return mul_rounded(ep.a3 * ep.dinv[i], u[i]);
     else return u[i];
   };
   float p = 0.0f, q = 0.0f, s = 0.0f, uo = 0.0f;
