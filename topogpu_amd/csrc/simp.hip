@@ -99,9 +99,17 @@ __global__ void k_density_chain(long long n, const double* __restrict__ rf, doub
 
 // Weighted sum over a batch of load cases (U holds nrhs contiguous slices of
 // 3 * nodes; w = NULL weighs every case 1): ce = sum_r w_r u_r,e^T KE u_r,e.
-__global__ void k_sensitivity(Grid g, int nrhs, const double* __restrict__ U, const double* __restrict__ w,
-                              const double* __restrict__ rp, const double* __restrict__ dh, double penal,
-                              double erange, double* ce_out, double* dcf) {
+// KE is staged in LDS and read there (every lane reads the same entry, a
+// broadcast): held in scalar registers, its 576 doubles overflow them and
+// the compiler spills them to vector-register lanes (2,300 extra
+// instructions per element).
+__global__ void __launch_bounds__(256) k_sensitivity(Grid g, int nrhs, const double* __restrict__ U,
+                                                    const double* __restrict__ w, const double* __restrict__ rp,
+                                                    const double* __restrict__ dh, double penal, double erange,
+                                                    double* ce_out, double* dcf) {
+  __shared__ double ske[576];
+  for (int t = threadIdx.x; t < 576; t += blockDim.x) ske[t] = c_ke64[t];
+  __syncthreads();
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= g.elems()) return;
   const int k = (int)(e % g.nz), j = (int)((e / g.nz) % g.ny), i = (int)(e / ((long long)g.nz * g.ny));
@@ -118,12 +126,28 @@ __global__ void k_sensitivity(Grid g, int nrhs, const double* __restrict__ U, co
       ue[3 * c + 2] = p[2];
     }
     double cr = 0.0;
+    // rows node by node, the node loop not unrolled: its KE rows are then
+    // read from LDS as they are needed instead of all 576 entries being
+    // hoisted into registers; the node's three values are picked by selects
+    // (an indexed register array would go to scratch memory)
+#pragma unroll 1
+    for (int na = 0; na < 8; ++na) {
+      double un[3];
 #pragma unroll
-    for (int a = 0; a < 24; ++a) {
-      double t = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        double v = ue[q];
 #pragma unroll
-      for (int b = 0; b < 24; ++b) t = fma(c_ke64[a * 24 + b], ue[b], t);
-      cr = fma(ue[a], t, cr);
+        for (int c = 1; c < 8; ++c) v = (na == c) ? ue[3 * c + q] : v;
+        un[q] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double* krow = ske + (3 * na + q) * 24;
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < 24; ++b) t = fma(krow[b], ue[b], t);
+        cr = fma(un[q], t, cr);
+      }
     }
     ce = fma(w ? w[r] : 1.0, cr, ce);
   }
