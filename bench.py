@@ -63,7 +63,8 @@ def _log(msg: str) -> None:
 # ``bench.py --kernel-probe``, scaled by the counts of a 256 MiB copy); the
 # summary file holds the commands and the figures.
 TRAFFIC_PROFILES = {"hex_bf16_mfma/CHEB_R": "profiles/hex_mfma_cheb_r_160x80x80.md",
-                    "hex_bf16x3_mfma/DOT": "profiles/hex_mfma_x3_dot_160x80x80.md"}
+                    "hex_bf16x3_mfma/DOT": "profiles/hex_mfma_x3_dot_160x80x80.md",
+                    "hex_bf16_mfma/CHEB_FIRST": "profiles/hex_mfma_cheb_first_160x80x80.md"}
 
 
 def profiled_traffic(kernel: str):
@@ -211,10 +212,10 @@ def saturated_kernel_times(eng, counter, reps: int = 50) -> dict:
 
 
 def kernel_probe(precision: str, reps: int = 20) -> dict:
-    """Reference copy of known size, then the two heaviest level-0 kernels on
-    live operands (the smoother's CHEB_R step and the PCG product, within a
-    few percent of each other): what a counter run scales its byte counts
-    with."""
+    """Reference copy of known size, then the three heaviest level-0 kernels
+    on live operands (the smoother's CHEB_R and CHEB_FIRST steps and the PCG
+    product, within a few percent of each other in device time per step):
+    what a counter run scales its byte counts with."""
     import torch
 
     from topogpu_amd.hip import lib as L
@@ -241,10 +242,15 @@ def kernel_probe(precision: str, reps: int = 20) -> dict:
     dot = torch.zeros(1, dtype=torch.float64, device=y.device)
     for _ in range(reps):
         L.call("tg_apply32", fine.fine_op(), L.ptr(u), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), st)
+    rhs = lv0.r.clone()
+    ep = L.ep(L.TG_EP_CHEB_FIRST, rhs=rhs, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1, a3=0.5)
+    for _ in range(reps):
+        L.call("tg_apply32", op, L.ptr(rhs), None, ep, st)
     torch.cuda.synchronize()
     return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "launches": reps,
             "kernels": {"k_hex_mfma<false, CHEB_R>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_R),
-                        "k_hex_mfma<true, DOT>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_DOT)}}
+                        "k_hex_mfma<true, DOT>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_DOT),
+                        "k_hex_mfma<false, CHEB_FIRST>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_FIRST)}}
 
 
 def matvec_bench(reps: int = 200, warmup: int = 5):
@@ -375,8 +381,8 @@ def main() -> int:
     ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="two warm-up SIMP iterations, then a 256 MiB device copy and 20 launches each of the "
-                         "level-0 BF16 CHEB_R smoother step and the split-BF16 DOT PCG product on the live "
-                         "operands (for hardware-counter runs)")
+                         "level-0 BF16 CHEB_R and CHEB_FIRST smoother steps and the split-BF16 DOT PCG product "
+                         "on the live operands (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode; run with --gpus 1", file=sys.stderr)

@@ -208,8 +208,8 @@ __global__ void __launch_bounds__(256) k_galerkin(Grid f, Grid c, const uint8_t*
 // Row-wise dense assembly: thread per DOF of the grid. The row's (at most
 // 27 x 3) nonzeros are summed in registers over the elements in corner order
 // and stored once each (A is zero beforehand).
-__global__ void k_assemble_dense(Grid g, const double* __restrict__ mats, const int* __restrict__ fidx, int nf,
-                                 double* __restrict__ A) {
+__global__ void __launch_bounds__(256) k_assemble_dense(Grid g, const double* __restrict__ mats,
+                                                        const int* __restrict__ fidx, int nf, double* __restrict__ A) {
   const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (d >= 3 * g.nodes()) return;
   const int row = fidx[d];
