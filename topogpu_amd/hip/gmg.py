@@ -77,7 +77,8 @@ class GMG:
                  l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree=2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
-                 nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32):
+                 nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32, smoother: str = "chebyshev",
+                 omega: float = 0.5, cycle: str = "v"):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
         if coarse_stencil not in (L.TG_OP_STENCIL_F32, L.TG_OP_STENCIL_BF16):
@@ -95,6 +96,15 @@ class GMG:
         self.coarse_degrees = [max(1, int(d)) for d in cd]
         self.frac = float(cheb_lower_frac)
         self.safety = float(cheb_safety)
+        # smoother_type / omega / cycle_type of GalerkinMatFreeGMG
+        # (multigrid_v4.py:473-511): Chebyshev-Jacobi or damped Jacobi with
+        # omega = min(omega, 0.9 * 2 / lambda_max of level 0) on every level
+        # (:705, :918-932, :963-967); V- or W-cycle (:1010-1042)
+        if smoother not in ("chebyshev", "jacobi"):
+            raise ValueError(f"smoother must be 'chebyshev' or 'jacobi'; got {smoother!r}")
+        if cycle not in ("v", "w"):
+            raise ValueError(f"cycle must be 'v' or 'w'; got {cycle!r}")
+        self.smoother, self.omega, self.cycle = smoother, float(omega), cycle
         self.power_iters = power_iters
         self.coarse_power_iters = coarse_power_iters
         self.dims = fem.level_dims(dims, n_levels)
@@ -259,7 +269,14 @@ class GMG:
     def _cheb_coeffs(self, lv: int, degree: int):
         """Per-step (a1, a2) of the recurrence alpha_0 = 1/sigma,
         alpha_k = 1/(sigma - delta^2 alpha_{k-1}/4), d_k = alpha_k z + alpha_k
-        delta^2 alpha_{k-1}/4 d_{k-1} (multigrid_v4.py:870-887)."""
+        delta^2 alpha_{k-1}/4 d_{k-1} (multigrid_v4.py:870-887).
+
+        Damped Jacobi is the same residual-update sequence with every step
+        d_k = omega D^-1 r_{k-1}: (omega, 0) for each of the degree steps."""
+        if self.smoother == "jacobi":
+            lam0 = self.levels[0].lam if np.isfinite(self.levels[0].lam) else 1.0
+            w = min(self.omega, 0.9 * 2.0 / max(lam0, 1e-6))
+            return [(w, 0.0)] * degree
         Lv = self.levels[lv]
         lmax = (Lv.lam if np.isfinite(Lv.lam) else 1.0) * self.safety
         lmin = self.frac * lmax
@@ -321,9 +338,18 @@ class GMG:
         L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
 
     def _vcycle_up(self, lv: int, stream):
-        """Second half: the coarser cycle, prolongation, post-smoothing."""
+        """Second half: the coarser cycle, prolongation, post-smoothing. A
+        W-cycle corrects twice from the coarser level (multigrid_v4.py:
+        1010-1042): x += P e_c and r -= A P e_c (CHEB_R with a1 = a2 = 0,
+        a3 = 1: no smoothing step), restrict, cycle again."""
         Lv, C = self.levels[lv], self.levels[lv + 1]
         ec = self._vcycle(lv + 1, C.rhs, stream)
+        if self.cycle == "w":
+            L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
+            self._launch(lv, self.op(lv, accurate=True), Lv.c, None,
+                         L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r, a1=0.0, a2=0.0, a3=1.0), stream)
+            L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
+            ec = self._vcycle(lv + 1, C.rhs, stream)
         L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
         self._postsmooth(lv, self._degree(lv), stream)
         return Lv.x
@@ -343,7 +369,8 @@ class GMG:
         the modulus buffers and the setup-time buffers."""
         return (tuple(float(Lv.lam) for Lv in self.levels), self.fine_degree, tuple(self.coarse_degrees),
                 self.safety, self.frac, L.ptr(self.E32), L.ptr(self.E64),
-                tuple(L.ptr(Lv.stencil) for Lv in self.levels), L.ptr(self.coarse_A), self.nrhs)
+                tuple(L.ptr(Lv.stencil) for Lv in self.levels), L.ptr(self.coarse_A), self.nrhs, self.smoother,
+                self.omega, self.cycle)
 
     def apply(self, r, stream):
         """z = M r for every right-hand side of the batch (r: nrhs slices of

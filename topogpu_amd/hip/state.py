@@ -84,7 +84,7 @@ class HipStateSolver:
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
-                 device=None):
+                 smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -118,7 +118,8 @@ class HipStateSolver:
         self.gmg = GMG(self.dims, mask, n_levels=n_levels, fine_op=ops[1], fine_accurate_op=ops[2],
                        l1_q8_op=ops[4], l1_q8_accurate_op=ops[3], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
-                       level1=level1, nrhs=self.nrhs, coarse_stencil=ops[5], device=self.dev)
+                       level1=level1, nrhs=self.nrhs, coarse_stencil=ops[5], smoother=smoother, omega=omega,
+                       cycle=cycle, device=self.dev)
         nb = self.nrhs * self.n
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.u = torch.zeros(nb, dtype=torch.float64, device=self.dev)

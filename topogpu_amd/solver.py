@@ -71,7 +71,8 @@ class SIMPSolver:
 
     Extra keyword options of the hip backend (all optional):
     precision ("bf16" | "bf16x3" | "fp32"), n_levels, fine_degree,
-    coarse_degree, inner_rtol, use_heaviside, beta, min_iter, conv_tol,
+    coarse_degree, smoother ("chebyshev" | "jacobi"), omega, cycle ("v" |
+    "w"), inner_rtol, use_heaviside, beta, min_iter, conv_tol,
     formulation ("filtered": the reference CPU algorithm, default;
     "solverv4": the reference GPU loop), weights (multi-load).
     """
