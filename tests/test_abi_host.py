@@ -132,3 +132,21 @@ def test_evidence_bundle(tmp_path):
     assert {r["path"] for r in rows} == {"history.csv", "summary.json", "rho_final.npy", "render_metadata.json"}
     assert all(len(r["sha256"]) == 64 for r in rows)
     assert b.root == tmp_path / "run"
+
+
+def test_checkpoint_files_roundtrip(tmp_path):
+    """Arrays in <path>.npz, scalars in <path>.json (pub_simp_solver.py:
+    397-410); nothing is loaded unless both files exist."""
+    from topogpu_amd.solver import load_checkpoint, save_checkpoint
+
+    path = str(tmp_path / "ckpt_4x3x2_case")
+    assert load_checkpoint(path) is None
+    rho = np.linspace(0.0, 1.0, 24)
+    save_checkpoint(path, {"rho": rho, "compliance_hist": np.array([3.0, 2.5]), "iteration": 2,
+                           "history": [{"iteration": 1}, {"iteration": 2}], "move": 0.2})
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["ckpt_4x3x2_case.json", "ckpt_4x3x2_case.npz"]
+    got = load_checkpoint(path)
+    assert np.array_equal(got["rho"], rho) and got["iteration"] == 2 and got["move"] == 0.2
+    assert list(got["compliance_hist"]) == [3.0, 2.5] and got["history"][1] == {"iteration": 2}
+    (tmp_path / "ckpt_4x3x2_case.json").unlink()
+    assert load_checkpoint(path) is None

@@ -184,3 +184,24 @@ def test_jacobi_smoother_and_cycles_match_host_restatement(gpu, cycle):
     assert st_.converged
     U_ref = oracle.solve_state(oracle.edof_table_3d(*dims), E, free, F, ndof)
     assert rel_l2(h(s.u), U_ref) < 1e-4
+
+
+def test_checkpoint_resume_equals_uninterrupted_run(gpu, tmp_path):
+    """checkpoint_dir / checkpoint_every (run_simp, pub_simp_solver.py:
+    495-516, 597-603): three iterations with a checkpoint at the third, then a
+    run to six that resumes from it, give the history and design of six
+    uninterrupted iterations."""
+    prob = tg.unit_cantilever((24, 12, 6), volfrac=0.30, rmin=1.5)
+    kw = dict(backend="hip", tol=1e-8, max_krylov=800, move=0.2)
+    full = tg.SIMPSolver(max_iter=6, **kw).solve(prob)
+    d = str(tmp_path / "ckpt")
+    first = tg.SIMPSolver(max_iter=3, checkpoint_dir=d, checkpoint_every=3, **kw).solve(prob)
+    assert len(first.history) == 3
+    assert (tmp_path / "ckpt" / f"ckpt_24x12x6_{prob.name}.npz").exists()
+    resumed = tg.SIMPSolver(max_iter=6, checkpoint_dir=d, checkpoint_every=3, **kw).solve(prob)
+    assert [r["iteration"] for r in resumed.history] == list(range(1, 7))
+    c_full = np.array([r["compliance"] for r in full.history])
+    c_res = np.array([r["compliance"] for r in resumed.history])
+    assert np.array_equal(c_full, c_res), c_res / c_full - 1
+    assert np.array_equal(full.rho_final, resumed.rho_final)
+    assert resumed.summary["n_iter"] == 6

@@ -17,6 +17,8 @@ serving them.
 
 from __future__ import annotations
 
+import json
+import os
 import time
 from dataclasses import dataclass
 from pathlib import Path
@@ -62,6 +64,31 @@ class OptimizationResult:
         return bundle
 
 
+def save_checkpoint(path: str, state: dict[str, Any]) -> None:
+    """Arrays to <path>.npz, everything else to <path>.json, each written to
+    a temporary name and renamed (_save_checkpoint, pub_simp_solver.py:
+    397-403)."""
+    arrays = {k: v for k, v in state.items() if isinstance(v, np.ndarray)}
+    scalars = {k: v for k, v in state.items() if not isinstance(v, np.ndarray)}
+    np.savez_compressed(path + ".tmp", **arrays)
+    os.replace(path + ".tmp.npz", path + ".npz")
+    with open(path + ".json.tmp", "w") as f:
+        json.dump(scalars, f)
+    os.replace(path + ".json.tmp", path + ".json")
+
+
+def load_checkpoint(path: str) -> dict[str, Any] | None:
+    """The state save_checkpoint wrote, or None (_load_checkpoint,
+    pub_simp_solver.py:405-410). Arrays load without pickles."""
+    if not (os.path.exists(path + ".npz") and os.path.exists(path + ".json")):
+        return None
+    with np.load(path + ".npz", allow_pickle=False) as z:
+        data: dict[str, Any] = {k: z[k] for k in z.files}
+    with open(path + ".json") as f:
+        data.update(json.load(f))
+    return data
+
+
 def grayness(rho: np.ndarray) -> float:
     return float(4.0 * np.mean(rho * (1.0 - rho)))
 
@@ -74,7 +101,12 @@ class SIMPSolver:
     coarse_degree, smoother ("chebyshev" | "jacobi"), omega, cycle ("v" |
     "w"), inner_rtol, use_heaviside, beta, min_iter, conv_tol,
     formulation ("filtered": the reference CPU algorithm, default;
-    "solverv4": the reference GPU loop), weights (multi-load).
+    "solverv4": the reference GPU loop), weights (multi-load),
+    checkpoint_dir and checkpoint_every (filtered formulation: every
+    checkpoint_every iterations the design, history and iteration count go
+    to <checkpoint_dir>/ckpt_<nx>x<ny>x<nz>_<problem name>.npz/.json, and a
+    run that finds them there resumes from them, as run_simp does,
+    pub_simp_solver.py:495-516, 597-603).
     """
 
     def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
@@ -112,6 +144,8 @@ class SIMPSolver:
         conv_tol = float(opts.pop("conv_tol", 1.0e-3))
         compliance_tol = float(opts.pop("compliance_tol", 5.0e-4))
         window = int(opts.pop("compliance_window", 5))
+        ckpt_dir = opts.pop("checkpoint_dir", None)
+        ckpt_every = int(opts.pop("checkpoint_every", 10))
         bc = generate_bc(spec)
         force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
         rmin = problem.filter_radius or spec.rmin or 1.5
@@ -121,7 +155,25 @@ class SIMPSolver:
         comp_hist: list[float] = []
         t_total = time.perf_counter()
         iteration = 0
-        for iteration in range(1, self.max_iter + 1):
+        start = 1
+        ckpt = None
+        if ckpt_dir is not None:
+            os.makedirs(ckpt_dir, exist_ok=True)
+            nx, ny, nz = problem.nel
+            ckpt = os.path.join(ckpt_dir, f"ckpt_{nx}x{ny}x{nz}_{problem.name}")
+            saved = load_checkpoint(ckpt)
+            if saved is not None:
+                # resume: the design, the history, the iteration count and,
+                # when saved, the displacements the next solve starts from
+                import torch
+                engine.rho.copy_(torch.from_numpy(np.asarray(saved["rho"], dtype=np.float64)))
+                if "u" in saved and saved["u"].size == engine.fea.u.numel():
+                    engine.fea.u.copy_(torch.from_numpy(np.asarray(saved["u"], dtype=np.float64)))
+                history = list(saved["history"])
+                comp_hist = [float(c) for c in saved["compliance_hist"]]
+                start = int(saved["iteration"]) + 1
+                iteration = start - 1
+        for iteration in range(start, self.max_iter + 1):
             t0 = time.perf_counter()
             step = engine.step()
             wall = time.perf_counter() - t0
@@ -136,6 +188,11 @@ class SIMPSolver:
             })
             win = comp_hist[-window:]
             conv_c = len(win) == window and abs(win[-1] - win[0]) / max(abs(win[0]), 1e-10) < compliance_tol
+            if ckpt is not None and iteration % ckpt_every == 0:
+                save_checkpoint(ckpt, {"rho": engine.rho.cpu().numpy(), "u": engine.fea.u.cpu().numpy(),
+                                       "compliance_hist": np.array(comp_hist),
+                                       "iteration": iteration, "history": history, "move": self.move,
+                                       "rmin": rmin})
             if step.change < conv_tol and conv_c and iteration >= min_iter:
                 break
         final_c, rho_final, _ = engine.evaluate()
