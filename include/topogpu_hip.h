@@ -287,15 +287,15 @@ int tg_graph_destroy(void* exec);
 /* ---- SIMP density / sensitivity / OC (FP64) ----------------------------- */
 
 /* out[i] = (sum_k w_k * s(in)[i+off_k]) * (out_scale ? out_scale[i] : 1),
- * s(in)[j] = in[j] * (in_scale ? in_scale[j] : 1); offsets (dx,dy,dz)
- * outside the grid are skipped. With in_scale = NULL and out_scale = 1/Hs
+ * s(in)[j] = in[j] * (in_scale ? in_scale[j] : 1); 1 <= m <= 256 offsets
+ * (dx,dy,dz), those outside the grid skipped. With in_scale = NULL and out_scale = 1/Hs
  * this is the density filter H (pub_simp_solver.py:215-243, H_mat.dot); with
  * in_scale = 1/Hs and out_scale = NULL it is H^T (:368, :371). */
 int tg_stencil64(const tg_grid* g, int32_t m, const int32_t* offs_host, const double* w_host,
                  const double* in, const double* in_scale, const double* out_scale,
                  double* out, void* stream);
 
-/* rho_phys = heaviside(rho_f, beta, eta) (identity if beta < 1e-6),
+/* rho_phys = heaviside(rho_f, beta, eta) (identity unless beta > 1e-6),
  * dh = d rho_phys / d rho_f, E = Emin + (E0-Emin) rho_phys^penal (FP64 and
  * FP32). pub_simp_solver.py:245-253, 347-350. Any output may be NULL. */
 int tg_density_chain(int64_t n, const double* rho_f, double beta, double eta, double penal,

@@ -30,6 +30,7 @@ from .simp import (  # noqa: F401
     oc_update_filtered,
     oc_update_mean,
     run_simp,
+    run_simp_controlled,
     assemble_k,
     solve_state,
 )

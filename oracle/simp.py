@@ -1,8 +1,9 @@
 """FP64 restatement of the reference CPU SIMP path (test infrastructure only).
 
 Reference: ``src/gpu_fem/pub_simp_solver.py`` of nbbllxx0/TopoGPU. Only the 3-D
-structured-hex8 path is restated; the 2-D presets, callbacks, tail phase and
-checkpointing of ``run_simp`` are outside the hot path.
+structured-hex8 path is restated: ``run_simp`` without and with a controller
+(best tracking, actions, tail phase); its 2-D presets and checkpoint files
+are not.
 """
 
 from __future__ import annotations
@@ -316,3 +317,100 @@ def run_simp(nelx, nely, nelz, volfrac, free, F, *, rmin=1.5, penal=3.0, move=0.
     return dict(compliance_history=hist, rho_final=rho_final, final_compliance=final_c,
                 final_grayness=float(4.0 * np.mean(rho_final * (1.0 - rho_final))),
                 n_iter=iteration, changes=changes, rho_design_history=rho_hist)
+
+
+def _grayness(rho):
+    return float(4.0 * np.mean(rho * (1.0 - rho)))
+
+
+def run_simp_controlled(nelx, nely, nelz, volfrac, free, F, *, callback, rmin=1.5, penal=3.0, move=0.2,
+                        max_iter=100, min_iter=50, tol=0.01, compliance_tol=5e-4, compliance_window=5,
+                        use_heaviside=True, beta=1.0, eta=0.5, min_penal_for_best=3.0,
+                        max_gray_for_best=0.25, tail_default_iters=18, beta_max=32.0, solver=None):
+    """``run_simp`` with a controller (pub_simp_solver.py:412-657, without
+    checkpoints): initial action, best / pre-valid tracking behind the
+    penal/grayness gate, per-iteration actions (clipped parameters, restart
+    from the best valid design, stop), the filter rebuilt at the top of an
+    iteration after rmin changed, the tail phase and the final evaluation
+    with the last parameters and the last built filter.
+
+    The controller sees ``state.iteration`` and the other StepState fields
+    the tests need; ``params`` carries ``tail_default_iters`` and
+    ``beta_max``."""
+    from types import SimpleNamespace
+
+    n = nelx * nely * nelz
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    edof = edof_table_3d(nelx, nely, nelz)
+    kw = dict(volfrac=volfrac, edof=edof, free=free, F=F, ndof=ndof, solver=solver)
+    params = SimpleNamespace(tail_default_iters=tail_default_iters, beta_max=beta_max, min_iter=min_iter)
+
+    def clip_action(act, p, r, m, b, rho_next, best_rho, valid):
+        # _apply_action, pub_simp_solver.py:379-386
+        if act.get("restart", False) and valid:
+            rho_next = best_rho.copy()
+        p = float(np.clip(act["penal"], 1.0, 5.0)) if "penal" in act else p
+        r = float(np.clip(act["rmin"], 1.1, 4.0)) if "rmin" in act else r
+        m = float(np.clip(act["move"], 0.03, 0.4)) if "move" in act else m
+        b = float(np.clip(act["beta"], 1.0, 64.0)) if "beta" in act else b
+        return p, r, m, b, rho_next
+
+    rho = volfrac * np.ones(n)
+    H = build_density_filter(nelx, nely, nelz, rmin)
+    h_rmin = rmin
+    if hasattr(callback, "initial_action"):
+        act = callback.initial_action(params)
+        if act:
+            penal, rmin, move, beta, rho = clip_action(act, penal, rmin, move, beta, rho, rho, False)
+    best = dict(rho=rho.copy(), c=np.inf, it=0, valid=False, gray=_grayness(rho))
+    pre = dict(rho=rho.copy(), c=np.inf, it=0, gray=_grayness(rho))
+    hist, log = [], []
+    iteration = 0
+    for iteration in range(1, max_iter + 1):
+        if abs(rmin - h_rmin) > 1e-6:
+            H, h_rmin = build_density_filter(nelx, nely, nelz, rmin), rmin
+        c, rho_new, change, rho_phys, _ = fea_and_sensitivity(rho, penal, move, beta, eta, use_heaviside, H, **kw)
+        hist.append(c)
+        gray = _grayness(rho_phys)
+        if c < best["c"] and penal >= min_penal_for_best and gray < max_gray_for_best:
+            best.update(rho=rho_phys.copy(), c=c, it=iteration, valid=True, gray=gray)
+        elif c < pre["c"] and not best["valid"] and iteration >= 10:
+            pre.update(rho=rho_phys.copy(), c=c, it=iteration, gray=gray)
+        win = hist[-compliance_window:]
+        conv = (change < tol and len(win) == compliance_window
+                and abs(win[-1] - win[0]) / max(abs(win[0]), 1e-10) < compliance_tol)
+        log.append(dict(phase="main", penal=penal, rmin=rmin, move=move, beta=beta, best_compliance=best["c"],
+                        best_is_valid=best["valid"]))
+        act = callback(SimpleNamespace(iteration=iteration, compliance=c, converged=conv,
+                                       best_is_valid=best["valid"]), rho_new)
+        if act:
+            penal, rmin, move, beta, rho_new = clip_action(act, penal, rmin, move, beta, rho_new, best["rho"],
+                                                           best["valid"])
+            if act.get("stop", False) and iteration >= min_iter:
+                rho = rho_new
+                break
+        rho = rho_new
+        if conv and iteration >= min_iter:
+            break
+    tail = {"enabled": False, "tail_iters": tail_default_iters, "restart_from_best": True,
+            "penal": 4.5, "rmin": 1.2, "move": 0.05, "beta": min(beta_max, 32.0)}
+    if hasattr(callback, "finalize_tail"):
+        tail.update(callback.finalize_tail(params) or {})
+    n_tail = 0
+    if tail["enabled"] and tail["tail_iters"] > 0:
+        if tail["restart_from_best"]:
+            rho = np.clip(best["rho"], 1e-3, 1.0) if best["valid"] else volfrac * np.ones(n)
+        penal, rmin = float(tail["penal"]), float(tail["rmin"])
+        move, beta = float(tail["move"]), float(tail["beta"])
+        H = build_density_filter(nelx, nely, nelz, rmin)
+        for n_tail in range(1, int(tail["tail_iters"]) + 1):
+            c, rho, _, _, _ = fea_and_sensitivity(rho, penal, move, beta, eta, use_heaviside, H, **kw)
+            hist.append(c)
+            log.append(dict(phase="tail", penal=penal, rmin=rmin, move=move, beta=beta,
+                            best_compliance=best["c"], best_is_valid=best["valid"]))
+    final_c, _, _, rho_final, _ = fea_and_sensitivity(rho, penal, move, beta, eta, use_heaviside, H, **kw)
+    return dict(compliance_history=hist, rho_final=rho_final, final_compliance=final_c,
+                final_grayness=_grayness(rho_final), n_iter=iteration + n_tail, best_rho=best["rho"],
+                best_compliance=best["c"], best_iteration=best["it"], best_is_valid=best["valid"],
+                best_grayness=best["gray"], pre_valid_best_compliance=pre["c"],
+                pre_valid_best_iteration=pre["it"], params_log=log)

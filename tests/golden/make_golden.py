@@ -3,7 +3,7 @@
 Run in a container where the reference sources are mounted read-only at
 /root/reference (never on the GPU box):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 It imports the reference's pure NumPy/SciPy CPU modules
 (``gpu_fem.pub_simp_solver``, ``gpu_fem.bc_generator``,
@@ -13,7 +13,9 @@ as small ``.npz`` fixtures next to this script. Only data is stored.
 
 Why an import: the composition of filter -> Heaviside -> solve -> sensitivity
 chain -> filtered-volume OC bisection over several iterations is not fixed by
-any closed form; the individual pieces are pinned by the oracle tests.
+any closed form; the individual pieces are pinned by the oracle tests. The
+controlled cases drive run_simp's continuation logic (actions, best
+tracking, tail) with the deterministic controllers of controllers.py.
 pyamg is absent here, so the reference solves with ``spsolve``
 (pub_simp_solver.py:44-49): the direct solve is the stricter oracle.
 """
@@ -47,6 +49,22 @@ CASES = {
                   loads=[dict(type="point", x=10.0, y=0.0, z=1.0, fy=-1.0)]),
         params=dict(rmin=2.0, max_iter=6, min_iter=2, move=0.2, tol=1e-3),
     ),
+    # run_simp with a controller (controllers.py): actions, best tracking
+    # and the tail phase on a 16x8x4 cantilever.
+    "c0c_controlled_16x8x4_A": dict(
+        spec=dict(Lx=16.0, Ly=8.0, Lz=4.0, nelx=16, nely=8, nelz=4, volfrac=0.30,
+                  supports=[dict(type="edge", edge="left", constraint="fixed")],
+                  loads=[dict(type="point", x=16.0, y=4.0, z=0.0, fz=-1.0)]),
+        params=dict(rmin=1.5, max_iter=12, min_iter=2, move=0.2, tol=1e-3, max_gray_for_best=0.9),
+        controller="A",
+    ),
+    "c0c_controlled_16x8x4_B": dict(
+        spec=dict(Lx=16.0, Ly=8.0, Lz=4.0, nelx=16, nely=8, nelz=4, volfrac=0.30,
+                  supports=[dict(type="edge", edge="left", constraint="fixed")],
+                  loads=[dict(type="point", x=16.0, y=4.0, z=0.0, fz=-1.0)]),
+        params=dict(rmin=1.5, max_iter=10, min_iter=2, move=0.2, tol=1e-3, beta_max=4.0),
+        controller="B",
+    ),
 }
 
 
@@ -56,13 +74,35 @@ def main() -> None:
     from gpu_fem.problem_spec import ProblemSpec
     from gpu_fem.pub_simp_solver import SIMPParams, run_simp
 
-    for name, case in CASES.items():
+    sys.path.insert(0, str(OUT))
+    from controllers import CONTROLLERS
+
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        case = CASES[name]
         spec = ProblemSpec.from_dict(case["spec"])
         bc = generate_bc(spec)
         p = case["params"]
         params = SIMPParams(nelx=spec.nelx, nely=spec.nely, nelz=spec.nelz,
                             volfrac=spec.volfrac, **p)
-        res = run_simp(params, verbose=False, bc_override=bc.bc_override)
+        ctrl = CONTROLLERS[case["controller"]]() if "controller" in case else None
+        res = run_simp(params, callback=ctrl, verbose=False, bc_override=bc.bc_override)
+        extra = {}
+        if ctrl is not None:
+            log = res["params_log"]
+            extra = dict(
+                best_rho=np.asarray(res["best_rho"], dtype=np.float64),
+                best_compliance=np.float64(res["best_compliance"]),
+                best_iteration=np.int64(res["best_iteration"]),
+                best_is_valid=np.int64(res["best_is_valid"]),
+                best_grayness=np.float64(res["best_grayness"]),
+                pre_valid_best_compliance=np.float64(res["pre_valid_best_compliance"]),
+                pre_valid_best_iteration=np.int64(res["pre_valid_best_iteration"]),
+                log_phase=np.array([r["phase"] == "tail" for r in log], dtype=np.int64),
+                log_controls=np.array([[r["penal"], r["rmin"], r["move"], r["beta"]] for r in log]),
+                log_best_compliance=np.array([r["best_compliance"] for r in log]),
+                log_best_is_valid=np.array([r["best_is_valid"] for r in log], dtype=np.int64),
+            )
         np.savez_compressed(
             OUT / f"{name}.npz",
             free=bc.free_dofs.astype(np.int64),
@@ -72,6 +112,7 @@ def main() -> None:
             final_compliance=np.float64(res["final_compliance"]),
             final_grayness=np.float64(res["final_grayness"]),
             n_iter=np.int64(res["n_iter"]),
+            **extra,
         )
         meta = dict(case=case, generator="tests/golden/make_golden.py",
                     reference="nbbllxx0/TopoGPU v0.1.0 src/gpu_fem/pub_simp_solver.py run_simp",

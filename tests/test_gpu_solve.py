@@ -186,22 +186,50 @@ def test_jacobi_smoother_and_cycles_match_host_restatement(gpu, cycle):
     assert rel_l2(h(s.u), U_ref) < 1e-4
 
 
-def test_checkpoint_resume_equals_uninterrupted_run(gpu, tmp_path):
+@pytest.mark.parametrize("controller", [None, "A"])
+def test_checkpoint_resume_equals_uninterrupted_run(gpu, tmp_path, controller):
     """checkpoint_dir / checkpoint_every (run_simp, pub_simp_solver.py:
-    495-516, 597-603): three iterations with a checkpoint at the third, then a
-    run to six that resumes from it, give the history and design of six
-    uninterrupted iterations."""
-    prob = tg.unit_cantilever((24, 12, 6), volfrac=0.30, rmin=1.5)
-    kw = dict(backend="hip", tol=1e-8, max_krylov=800, move=0.2)
-    full = tg.SIMPSolver(max_iter=6, **kw).solve(prob)
+    495-516, 597-603): a run stopped after a checkpoint, then a run that
+    resumes from it, give the history and design of one uninterrupted run --
+    also under a controller, whose parameters and best design the checkpoint
+    carries."""
+    from golden.controllers import CONTROLLERS
+
+    ctl = (lambda: CONTROLLERS[controller]()) if controller else (lambda: None)
+    dims, n, every = ((24, 12, 6), 6, 3) if controller is None else ((16, 8, 4), 12, 4)
+    prob = tg.unit_cantilever(dims, volfrac=0.30, rmin=1.5)
+    kw = dict(backend="hip", tol=1e-8, max_krylov=800, move=0.2, max_gray_for_best=0.9)
+    full = tg.SIMPSolver(max_iter=n, callback=ctl(), **kw).solve(prob)
     d = str(tmp_path / "ckpt")
-    first = tg.SIMPSolver(max_iter=3, checkpoint_dir=d, checkpoint_every=3, **kw).solve(prob)
-    assert len(first.history) == 3
-    assert (tmp_path / "ckpt" / f"ckpt_24x12x6_{prob.name}.npz").exists()
-    resumed = tg.SIMPSolver(max_iter=6, checkpoint_dir=d, checkpoint_every=3, **kw).solve(prob)
-    assert [r["iteration"] for r in resumed.history] == list(range(1, 7))
+    first = tg.SIMPSolver(max_iter=every, checkpoint_dir=d, checkpoint_every=every, callback=ctl(), **kw).solve(prob)
+    assert first.history[every - 1]["iteration"] == every
+    assert (tmp_path / "ckpt" / f"ckpt_{dims[0]}x{dims[1]}x{dims[2]}_{prob.name}.npz").exists()
+    resumed = tg.SIMPSolver(max_iter=n, checkpoint_dir=d, checkpoint_every=every, callback=ctl(), **kw).solve(prob)
+    assert [r["iteration"] for r in resumed.history] == [r["iteration"] for r in full.history]
     c_full = np.array([r["compliance"] for r in full.history])
     c_res = np.array([r["compliance"] for r in resumed.history])
     assert np.array_equal(c_full, c_res), c_res / c_full - 1
     assert np.array_equal(full.rho_final, resumed.rho_final)
-    assert resumed.summary["n_iter"] == 6
+    assert resumed.summary["n_iter"] == full.summary["n_iter"]
+    assert resumed.summary["best_iteration"] == full.summary["best_iteration"]
+    assert np.array_equal(full.best_rho, resumed.best_rho)
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_controlled_run_matches_reference_golden(gpu, golden, case):
+    """run_simp with a controller (pub_simp_solver.py:412-657) on the GPU:
+    actions, best tracking, restarts, stop and tail against the reference's
+    own run with the same controller (16x8x4 cantilever)."""
+    from golden.controllers import CONTROLLERS
+    from test_oracle import CONTROLLED, check_controlled_run
+
+    g = golden(f"c0c_controlled_16x8x4_{case}")
+    prob = tg.unit_cantilever((16, 8, 4), volfrac=0.30, rmin=1.5)
+    kw = dict(CONTROLLED[case])
+    res = tg.SIMPSolver(backend="hip", tol=1e-8, max_krylov=800, move=0.2, max_iter=kw.pop("max_iter"),
+                        callback=CONTROLLERS[case](), **kw).solve(prob)
+    s = res.summary
+    comp = [r["compliance"] for r in res.history]
+    assert [r["iteration"] for r in res.history] == list(range(1, s["n_iter"] + 1))
+    check_controlled_run(g, comp, res.rho_final, s["final_compliance"], s["n_iter"], dict(s, best_rho=res.best_rho),
+                         s["params_log"], rtol=1e-6, rho_tol=1e-5)

@@ -213,6 +213,46 @@ def test_run_simp_matches_reference_golden(golden, name):
     assert r["final_compliance"] == pytest.approx(float(g["final_compliance"]), rel=1e-9)
 
 
+CONTROLLED = {"A": dict(max_iter=12, max_gray_for_best=0.9), "B": dict(max_iter=10, beta_max=4.0)}
+
+
+def check_controlled_run(g, comp, rho_final, final_c, n_iter, best, log, rtol, rho_tol):
+    """A controlled run against a golden one: the trajectory, the best design
+    bookkeeping and the per-iteration controls and phases."""
+    assert n_iter == int(g["n_iter"])
+    assert np.allclose(comp, g["compliance_history"], rtol=rtol, atol=0), np.asarray(comp) / g["compliance_history"] - 1
+    assert np.abs(rho_final - g["rho_final"]).max() < rho_tol
+    assert final_c == pytest.approx(float(g["final_compliance"]), rel=rtol)
+    assert best["best_is_valid"] == bool(g["best_is_valid"])
+    assert best["best_iteration"] == int(g["best_iteration"])
+    assert best["pre_valid_best_iteration"] == int(g["pre_valid_best_iteration"])
+    for k in ("best_compliance", "best_grayness", "pre_valid_best_compliance"):
+        gv = float(g[k])
+        assert (best[k] == gv) if np.isinf(gv) else best[k] == pytest.approx(gv, rel=rtol), k
+    assert np.abs(best["best_rho"] - g["best_rho"]).max() < rho_tol
+    assert [r["phase"] == "tail" for r in log] == [bool(v) for v in g["log_phase"]]
+    assert np.array_equal([[r["penal"], r["rmin"], r["move"], r["beta"]] for r in log], g["log_controls"])
+    assert [bool(r["best_is_valid"]) for r in log] == [bool(v) for v in g["log_best_is_valid"]]
+    # inf (no valid best yet) must match inf exactly; allclose compares it so
+    lb = np.array([r["best_compliance"] for r in log])
+    assert np.allclose(lb, g["log_best_compliance"], rtol=rtol, atol=0)
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_run_simp_controlled_matches_reference_golden(golden, case):
+    """run_simp with a controller (pub_simp_solver.py:412-657): initial and
+    per-iteration actions with clipping, the validity gate, restarts, a stop,
+    the filter rebuilt after an rmin change and the tail phase, against the
+    reference's own run with the same controller."""
+    from golden.controllers import CONTROLLERS
+
+    g = golden(f"c0c_controlled_16x8x4_{case}")
+    r = oracle.run_simp_controlled(16, 8, 4, 0.30, g["free"], g["F"], callback=CONTROLLERS[case](), rmin=1.5,
+                                   move=0.2, min_iter=2, tol=1e-3, **CONTROLLED[case])
+    check_controlled_run(g, r["compliance_history"], r["rho_final"], r["final_compliance"], r["n_iter"], r,
+                         r["params_log"], rtol=1e-9, rho_tol=1e-9)
+
+
 @pytest.mark.parametrize("name", ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2"])
 def test_generate_bc_matches_reference_golden(golden, name):
     """Fixed/free DOFs and the load vector equal those the reference's own

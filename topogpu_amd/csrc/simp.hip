@@ -13,7 +13,9 @@
 namespace tg {
 namespace {
 
-constexpr int kMaxOffs = 128;
+// rmin up to 4 (the largest radius a continuation action may set): 251
+// offsets
+constexpr int kMaxOffs = 256;
 struct Stencil {
   int m;
   int8_t off[kMaxOffs][3];
@@ -84,7 +86,7 @@ __global__ void k_density_chain(long long n, const double* __restrict__ rf, doub
   if (e >= n) return;
   const double x = rf[e];
   double p = x, d = 1.0;
-  if (beta >= 1e-6) {
+  if (beta > 1e-6) {
     const double den = tanh_d(beta * eta) + tanh_d(beta * (1.0 - eta));
     const double th = tanh_d(beta * (x - eta));
     p = (tanh_d(beta * eta) + th) / den;

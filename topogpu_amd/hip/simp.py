@@ -59,11 +59,7 @@ class HipSIMP:
         self.weights_host = w.copy()
         self.weights = torch.from_numpy(w.copy()).to(self.dev)
         self.grid = self.fea.grid
-        offs, w = fem.filter_stencil(self.rmin)
-        self.offs = np.ascontiguousarray(offs.ravel(), dtype=np.int32)
-        self.w = np.ascontiguousarray(w, dtype=np.float64)
-        self.m = int(w.size)
-        self.hs_inv = torch.from_numpy(1.0 / self._row_sums()).to(self.dev)
+        self.set_rmin(self.rmin)
         f64 = dict(dtype=torch.float64, device=self.dev)
         rho = np.full(self.n_elem, self.volfrac) if rho0 is None else np.asarray(rho0, dtype=np.float64)
         self.rho = torch.from_numpy(rho.copy()).to(self.dev)
@@ -81,6 +77,16 @@ class HipSIMP:
         self.hi = torch.zeros(self.n_elem, **f64)
         self.oc_state = torch.zeros(max(L.OC_STATE_LEN, L.OC_MEAN_STATE_LEN), **f64)
         self.scal = torch.zeros(4, **f64)
+
+    def set_rmin(self, rmin: float) -> None:
+        """(Re)build the density filter for radius rmin: its offsets and
+        weights, and the inverse row sums 1 / Hs."""
+        self.rmin = float(rmin)
+        offs, w = fem.filter_stencil(self.rmin)
+        self.offs = np.ascontiguousarray(offs.ravel(), dtype=np.int32)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        self.m = int(w.size)
+        self.hs_inv = torch.from_numpy(1.0 / self._row_sums()).to(self.dev)
 
     def _row_sums(self) -> np.ndarray:
         """Hs = W 1 on the host (geometry only)."""

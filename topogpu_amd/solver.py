@@ -50,6 +50,7 @@ class OptimizationResult:
     rho_final: np.ndarray
     history: list[dict[str, Any]]
     summary: dict[str, Any]
+    best_rho: np.ndarray | None = None   # best valid physical density (run_simp's best_rho)
 
     def save(self, path) -> EvidenceBundle:
         """Evidence folder (src/topogpu/solver.py:46-61)."""
@@ -106,7 +107,9 @@ class SIMPSolver:
     checkpoint_every iterations the design, history and iteration count go
     to <checkpoint_dir>/ckpt_<nx>x<ny>x<nz>_<problem name>.npz/.json, and a
     run that finds them there resumes from them, as run_simp does,
-    pub_simp_solver.py:495-516, 597-603).
+    pub_simp_solver.py:495-516, 597-603), callback (run_simp's controller,
+    topogpu_amd.control), min_penal_for_best, max_gray_for_best,
+    tail_default_iters, beta_max, seed.
     """
 
     def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
@@ -130,6 +133,13 @@ class SIMPSolver:
         raise ValueError(f"Unknown backend {self.backend!r}.")
 
     def _solve_hip(self, problem: TopologyProblem) -> OptimizationResult:
+        """run_simp (pub_simp_solver.py:412-657) with the step on the GPU:
+        the main loop with best-design tracking, the optional controller,
+        checkpoints and convergence test, then the optional tail phase and
+        the final re-evaluation."""
+        import torch
+
+        from . import control as C
         from .hip.simp import HipSIMP
 
         if self.linear_solver not in {"auto", "pcg_gmg"}:
@@ -146,64 +156,144 @@ class SIMPSolver:
         window = int(opts.pop("compliance_window", 5))
         ckpt_dir = opts.pop("checkpoint_dir", None)
         ckpt_every = int(opts.pop("checkpoint_every", 10))
+        callback = opts.pop("callback", None)
+        seed = opts.pop("seed", None)
+        min_penal_best = float(opts.pop("min_penal_for_best", 3.0))
+        max_gray_best = float(opts.pop("max_gray_for_best", 0.25))
+        tail_default_iters = int(opts.pop("tail_default_iters", 18))
+        beta_max = float(opts.pop("beta_max", 32.0))
+        if seed is not None:
+            np.random.seed(seed)
         bc = generate_bc(spec)
         force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
         rmin = problem.filter_radius or spec.rmin or 1.5
         engine = HipSIMP(problem.nel, bc.free_dofs, force, volfrac=spec.volfrac, rmin=rmin, move=self.move,
                          tol=self.tol, max_krylov=self.max_krylov, **opts)
-        history: list[dict[str, Any]] = []
-        comp_hist: list[float] = []
-        t_total = time.perf_counter()
-        iteration = 0
-        start = 1
-        ckpt = None
-        if ckpt_dir is not None:
-            os.makedirs(ckpt_dir, exist_ok=True)
-            nx, ny, nz = problem.nel
-            ckpt = os.path.join(ckpt_dir, f"ckpt_{nx}x{ny}x{nz}_{problem.name}")
-            saved = load_checkpoint(ckpt)
-            if saved is not None:
-                # resume: the design, the history, the iteration count and,
-                # when saved, the displacements the next solve starts from
-                import torch
-                engine.rho.copy_(torch.from_numpy(np.asarray(saved["rho"], dtype=np.float64)))
-                if "u" in saved and saved["u"].size == engine.fea.u.numel():
-                    engine.fea.u.copy_(torch.from_numpy(np.asarray(saved["u"], dtype=np.float64)))
-                history = list(saved["history"])
-                comp_hist = [float(c) for c in saved["compliance_hist"]]
-                start = int(saved["iteration"]) + 1
-                iteration = start - 1
-        for iteration in range(start, self.max_iter + 1):
+        nx, ny, nz = problem.nel
+        params = C.RunParams(nelx=nx, nely=ny, nelz=nz, volfrac=spec.volfrac, penal=engine.penal, rmin=rmin,
+                             move=self.move, max_iter=self.max_iter, tol=conv_tol, compliance_tol=compliance_tol,
+                             compliance_window=window, min_iter=min_iter, tail_default_iters=tail_default_iters,
+                             use_heaviside=engine.use_heaviside, beta_init=engine.beta, beta_max=beta_max,
+                             eta=engine.eta, min_penal_for_best=min_penal_best, max_gray_for_best=max_gray_best,
+                             seed=seed, checkpoint_dir=ckpt_dir, checkpoint_every=ckpt_every)
+        ctl = C.Controls(engine.penal, rmin, self.move, engine.beta)
+        rho0 = engine.rho.cpu().numpy()
+        if callback is not None and hasattr(callback, "initial_action"):
+            init = callback.initial_action(params)
+            if init:
+                C.apply_action(init, ctl, rho0, rho0, False)
+        best = C.BestTracker(rho0, grayness(rho0), min_penal_best, max_gray_best)
+
+        def to_device(rho: np.ndarray) -> None:
+            engine.rho.copy_(torch.from_numpy(np.ascontiguousarray(rho, dtype=np.float64)))
+
+        def run_step(it: int, phase: str) -> tuple[Any, np.ndarray, float]:
+            engine.penal, engine.move, engine.beta = ctl.penal, ctl.move, ctl.beta
             t0 = time.perf_counter()
             step = engine.step()
             wall = time.perf_counter() - t0
             comp_hist.append(step.compliance)
             rho_phys = engine.rho_phys.cpu().numpy()
+            gray = grayness(rho_phys)
             history.append({
-                "iteration": iteration, "compliance": float(step.compliance), "outer_iters": int(step.iters),
+                "iteration": it, "compliance": float(step.compliance), "outer_iters": int(step.iters),
                 "rho_mean": float(rho_phys.mean()), "rho_min": float(rho_phys.min()),
-                "rho_max": float(rho_phys.max()), "grayness": grayness(rho_phys), "wall_s": wall,
+                "rho_max": float(rho_phys.max()), "grayness": gray, "wall_s": wall,
                 "outer_solver": "pcg", "change": float(step.change), "ir_restarts": int(step.restarts),
                 "rel_residual": float(step.rel_residual), "converged": bool(step.converged),
             })
+            params_log.append({"iter": it, "phase": phase, "penal": ctl.penal, "rmin": ctl.rmin, "move": ctl.move,
+                               "beta": ctl.beta, "compliance": float(step.compliance),
+                               "best_compliance": float(best.compliance), "grayness": gray,
+                               "best_is_valid": best.is_valid,
+                               "volume_fraction": float(rho_phys.sum() / rho_phys.size)})
+            return step, rho_phys, gray
+
+        history: list[dict[str, Any]] = []
+        comp_hist: list[float] = []
+        params_log: list[dict[str, Any]] = []
+        t_total = time.perf_counter()
+        start = 1
+        ckpt = None
+        if ckpt_dir is not None:
+            os.makedirs(ckpt_dir, exist_ok=True)
+            ckpt = os.path.join(ckpt_dir, f"ckpt_{nx}x{ny}x{nz}_{problem.name}")
+            saved = load_checkpoint(ckpt)
+            if saved is not None:
+                # resume (pub_simp_solver.py:498-514): the design, the best
+                # design, the controls, the history and, when saved, the
+                # displacements the next solve starts from
+                to_device(saved["rho"])
+                if "u" in saved and saved["u"].size == engine.fea.u.numel():
+                    engine.fea.u.copy_(torch.from_numpy(np.asarray(saved["u"], dtype=np.float64)))
+                best.restore(saved)
+                ctl.penal, ctl.rmin = float(saved["penal"]), float(saved["rmin"])
+                ctl.move, ctl.beta = float(saved["move"]), float(saved["beta"])
+                comp_hist = [float(c) for c in saved.get("compliance_hist", [])]
+                history = list(saved.get("history", []))
+                start = int(saved["iteration"]) + 1
+        iteration = start - 1
+        for iteration in range(start, self.max_iter + 1):
+            if abs(ctl.rmin - engine.rmin) > 1e-6:
+                engine.set_rmin(ctl.rmin)
+            step, rho_phys, gray = run_step(iteration, "main")
+            # the log row holds the best after this iteration's update
+            # (pub_simp_solver.py:525-546, 576-581)
+            best.update(iteration, step.compliance, rho_phys, gray, ctl.penal)
+            params_log[-1]["best_compliance"] = float(best.compliance)
+            params_log[-1]["best_is_valid"] = best.is_valid
             win = comp_hist[-window:]
             conv_c = len(win) == window and abs(win[-1] - win[0]) / max(abs(win[0]), 1e-10) < compliance_tol
+            conv = step.change < conv_tol and conv_c
+            if callback is not None:
+                rho_new = engine.rho.cpu().numpy()
+                state = C.step_state(iteration, comp_hist, window, conv, rho_phys, gray, (nx, ny, nz), best, ctl)
+                action = callback(state, rho_new)
+                if action:
+                    rho_new = C.apply_action(action, ctl, rho_new, best.rho, best.is_valid)
+                # the next design is the controller's (it may also have
+                # changed the array in place)
+                to_device(rho_new)
+                if action and action.get("stop", False) and iteration >= min_iter:
+                    break
             if ckpt is not None and iteration % ckpt_every == 0:
-                save_checkpoint(ckpt, {"rho": engine.rho.cpu().numpy(), "u": engine.fea.u.cpu().numpy(),
-                                       "compliance_hist": np.array(comp_hist),
-                                       "iteration": iteration, "history": history, "move": self.move,
-                                       "rmin": rmin})
-            if step.change < conv_tol and conv_c and iteration >= min_iter:
+                state = {"rho": engine.rho.cpu().numpy(), "u": engine.fea.u.cpu().numpy(),
+                         "compliance_hist": np.array(comp_hist), "penal": ctl.penal, "rmin": ctl.rmin,
+                         "move": ctl.move, "beta": ctl.beta, "iteration": iteration, "history": history}
+                state.update(best.checkpoint())
+                save_checkpoint(ckpt, state)
+            if conv and iteration >= min_iter:
                 break
+
+        tail = C.tail_defaults(callback, params)
+        tail_run = 0
+        if tail.get("enabled", False) and tail.get("tail_iters", 0) > 0:
+            # pub_simp_solver.py:612-633: restart from the best valid design
+            # (clipped to [1e-3, 1]) or, with none, from the uniform start
+            if tail.get("restart_from_best", True):
+                to_device(np.clip(best.rho, 1e-3, 1.0) if best.is_valid else np.full(engine.n_elem, spec.volfrac))
+            ctl.penal, ctl.rmin = float(tail.get("penal", ctl.penal)), float(tail.get("rmin", ctl.rmin))
+            ctl.move, ctl.beta = float(tail.get("move", ctl.move)), float(tail.get("beta", ctl.beta))
+            engine.set_rmin(ctl.rmin)
+            for k in range(1, int(tail["tail_iters"]) + 1):
+                run_step(iteration + k, "tail")
+                tail_run = k
+        # final re-evaluation with the last controls (the filter is the one
+        # last built, pub_simp_solver.py:635-636)
+        engine.penal, engine.beta = ctl.penal, ctl.beta
         final_c, rho_final, _ = engine.evaluate()
         summary = {
             "problem": problem.to_dict(), "backend": "hip",
             "linear_solver": "pcg_gmg" if self.linear_solver == "auto" else self.linear_solver,
-            "precision": engine.fea.precision, "n_levels": len(engine.fea.gmg.levels), "n_iter": iteration,
-            "total_wall_s": time.perf_counter() - t_total, "final_compliance": float(final_c),
-            "final_grayness": grayness(rho_final), "final": history[-1] if history else {},
+            "precision": engine.fea.precision, "n_levels": len(engine.fea.gmg.levels),
+            "n_iter": iteration + tail_run, "total_wall_s": time.perf_counter() - t_total,
+            "final_compliance": float(final_c), "final_grayness": grayness(rho_final),
+            "final": history[-1] if history else {},
         }
-        return OptimizationResult(problem, rho_final, history, summary)
+        summary.update(best.summary())
+        if callback is not None:
+            summary.update({"n_tail_iters": tail_run, "tail_config": tail, "params_log": params_log})
+        return OptimizationResult(problem, rho_final, history, summary, best_rho=best.rho)
 
 
     def _solve_hip_solverv4(self, problem: TopologyProblem, opts: dict[str, Any]) -> OptimizationResult:
