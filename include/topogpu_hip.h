@@ -251,6 +251,17 @@ int tg_pcg_dir(int64_t n, float* p, const float* z, const double* zq, const doub
                void* stream);
 /* x64 += double(d32): iterative-refinement update of the FP64 iterate. */
 int tg_axpy_f32_to_f64(int64_t n, double* x64, const float* d32, void* stream);
+/* y = (accumulate ? y : 0) + a * x with a = scale * (alpha ? *alpha : 1) and
+ * x = x64, or double(x32) when x64 is NULL; alpha is a device scalar. The
+ * FGMRES vector updates of _cupy_fgmres (multigrid_v4.py:387 Gram-Schmidt
+ * w -= H[i,j] V[i] with scale -1; :438 x += y_j z_j; z_j = M_op(V[j]) of
+ * :380 taken to FP64 with accumulate = 0). */
+int tg_axpy64(int64_t n, double* y, const double* x64, const float* x32, const double* alpha,
+              double scale, int32_t accumulate, void* stream);
+/* y = x / sqrt(*ss) and y32 = float(y) (either may be NULL; 0 when
+ * *ss <= 0): the new Krylov vector w / ||w|| of _cupy_fgmres
+ * (multigrid_v4.py:365, 388-392) with *ss = w.w from tg_dot64. */
+int tg_normalize64(int64_t n, const double* x, const double* ss, double* y, float* y32, void* stream);
 /* First Chebyshev step from a zero iterate: d = x = a * dinv * rhs and, when
  * r != NULL, r = rhs (_chebyshev_steps, multigrid_v4.py:874-879, A x0 = 0). */
 int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, float* d, float* x,

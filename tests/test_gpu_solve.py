@@ -233,3 +233,43 @@ def test_controlled_run_matches_reference_golden(gpu, golden, case):
     assert [r["iteration"] for r in res.history] == list(range(1, s["n_iter"] + 1))
     check_controlled_run(g, comp, res.rho_final, s["final_compliance"], s["n_iter"], dict(s, best_rho=res.best_rho),
                          s["params_log"], rtol=1e-6, rho_tol=1e-5)
+
+
+@pytest.mark.parametrize("restart", [32, 5])
+@pytest.mark.parametrize("dims,levels", [((16, 8, 8), 3), ((13, 7, 9), 3)])
+def test_fgmres_state_solve_matches_direct(gpu, dims, levels, restart):
+    """outer_solver="fgmres" (_cupy_fgmres, multigrid_v4.py:325-443): FP64
+    right-preconditioned FGMRES around the BF16 V-cycle, with and without
+    restarts, against the direct solve."""
+    free, F, ndof = cantilever_bc(dims, point_support=True)
+    rng = np.random.default_rng(22)
+    E = 1e-9 + rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3
+    U_ref = oracle.solve_state(oracle.edof_table_3d(*dims), E, free, F, ndof)
+    s = HipStateSolver(dims, free, F, precision="bf16", n_levels=levels, tol=1e-9, max_krylov=400,
+                       outer_solver="fgmres", restart=restart)
+    E64 = torch.from_numpy(E).to(DEV)
+    s.set_modulus(E64, E64.float(), float(E.max()))
+    st = s.solve()
+    assert st.converged, st
+    if restart == 5:
+        assert st.restarts > 2            # restart cycles were taken
+    assert st.rel_residual <= 1e-8       # the Givens estimate tracks the true FP64 residual
+    U = h(s.u)
+    assert rel_l2(U, U_ref) < 1e-6
+    assert s.compliance() == pytest.approx(float(F @ U_ref), rel=1e-8)
+    st2 = s.solve()
+    assert st2.converged and st2.iters <= 2
+    with pytest.raises(ValueError):
+        HipStateSolver(dims, free, np.stack([F, F]), outer_solver="fgmres")
+
+
+def test_simp_c0_with_fgmres_matches_reference_golden(gpu, golden):
+    """The golden 24x12x6 trajectory with the FGMRES outer solver."""
+    g = golden("c0_cantilever_24x12x6")
+    prob = tg.unit_cantilever((24, 12, 6), volfrac=0.30, rmin=1.5)
+    res = tg.SIMPSolver(backend="hip", tol=1e-8, max_krylov=800, max_iter=3, move=0.2,
+                        outer_solver="fgmres").solve(prob)
+    comp = np.array([r["compliance"] for r in res.history])
+    assert all(r["outer_solver"] == "fgmres" for r in res.history)
+    assert np.allclose(comp, g["compliance_history"], rtol=1e-6, atol=0), comp / g["compliance_history"] - 1
+    assert np.abs(res.rho_final - g["rho_final"]).max() < 1e-5

@@ -259,12 +259,56 @@ __global__ void k_axpy_f32_to_f64(long long n, double* x, const float* d) {
     x[i] += (double)d[i];
 }
 
+// y = (accumulate ? y : 0) + a * x, a = scale * (alpha ? *alpha : 1), x read
+// from x64 or, when x64 is NULL, from x32. alpha is a device scalar written
+// by an earlier kernel on the stream (a Gram-Schmidt coefficient).
+__global__ void k_axpy64(long long n, double* __restrict__ y, const double* __restrict__ x64,
+                         const float* __restrict__ x32, const double* __restrict__ alpha, double scale,
+                         int accumulate) {
+  const double a = alpha ? scale * *alpha : scale;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double xv = x64 ? x64[i] : (double)x32[i];
+    y[i] = accumulate ? y[i] + a * xv : a * xv;
+  }
+}
+
+// y = x / sqrt(*ss) (0 when *ss <= 0), y32 = float(y).
+__global__ void k_normalize64(long long n, const double* __restrict__ x, const double* __restrict__ ss,
+                              double* __restrict__ y, float* __restrict__ y32) {
+  const double v = *ss;
+  const double h = v > 0.0 ? sqrt(v) : 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double q = h > 0.0 ? x[i] / h : 0.0;
+    if (y) y[i] = q;
+    if (y32) y32[i] = (float)q;
+  }
+}
+
 }  // namespace
 }  // namespace tg
 
 using namespace tg;
 
 // ---------------------------------------------------------------- C ABI
+
+extern "C" int tg_axpy64(int64_t n, double* y, const double* x64, const float* x32, const double* alpha,
+                         double scale, int32_t accumulate, void* s) {
+  TG_REQUIRE(n > 0 && y && (x64 || x32), "axpy64: bad argument");
+  hipLaunchKernelGGL(k_axpy64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, (hipStream_t)s, (long long)n, y, x64, x32,
+                     alpha, scale, (int)accumulate);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
+extern "C" int tg_normalize64(int64_t n, const double* x, const double* ss, double* y, float* y32, void* s) {
+  TG_REQUIRE(n > 0 && x && ss && (y || y32), "normalize64: bad argument");
+  hipLaunchKernelGGL(k_normalize64, dim3(vec_blocks(n)), dim3(kVecBlock), 0, (hipStream_t)s, (long long)n, x, ss, y,
+                     y32);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
 
 extern "C" const char* tg_last_error(void) { return g_err; }
 extern "C" int tg_version(void) { return 1; }

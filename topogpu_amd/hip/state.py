@@ -84,7 +84,8 @@ class HipStateSolver:
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
-                 smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", device=None):
+                 smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", outer_solver: str = "pcg",
+                 restart: int = 32, device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -106,6 +107,14 @@ class HipStateSolver:
         if F.ndim != 2 or F.shape[1] != self.n or not 1 <= F.shape[0] <= 64:
             raise ValueError(f"F must have shape ({self.n},) or (nrhs, {self.n}) with 1 <= nrhs <= 64")
         self.nrhs = int(F.shape[0])
+        if outer_solver not in ("pcg", "fgmres"):
+            raise ValueError(f"outer_solver must be 'pcg' or 'fgmres'; got {outer_solver!r}")
+        if outer_solver == "fgmres" and self.nrhs != 1:
+            raise ValueError("outer_solver='fgmres' solves one load case at a time (nrhs = 1)")
+        if int(restart) < 1:
+            raise ValueError("restart must be >= 1")
+        self.outer_solver, self.restart = outer_solver, int(restart)
+        self._krylov = None
         F[:, np.setdiff1d(np.arange(self.n), free)] = 0.0
         self.F = torch.from_numpy(F.ravel()).to(self.dev)
         self.bnorms = np.linalg.norm(F, axis=1)
@@ -167,6 +176,8 @@ class HipStateSolver:
         return np.sqrt(np.maximum(self.s[4].cpu().numpy(), 0.0))
 
     def solve(self) -> SolveStats:
+        if self.outer_solver == "fgmres":
+            return self._solve_fgmres()
         st, n, nr = self.stream, self.n, self.nrhs
         if not self.warm_start:
             L.call("tg_fill64", nr * n, L.ptr(self.u), 0.0, st)
@@ -207,6 +218,99 @@ class HipStateSolver:
         return self.last
 
     # ------------------------------------------------------------------
+    def _solve_fgmres(self) -> SolveStats:
+        """The reference's other outer solver: right-preconditioned restarted
+        FGMRES in FP64 with the GMG V-cycle as the (variable) preconditioner
+        (_cupy_fgmres, multigrid_v4.py:325-443, chosen by solver_v4.py:
+        187-190, 214-233): from the warm start, and once more from zero when
+        that does not converge. The Krylov vectors stay on the device; the
+        modified Gram-Schmidt coefficients are device scalars consumed by the
+        next update, and the host reads one Hessenberg column per step for
+        the Givens rotations and the convergence test."""
+        st, n = self.stream, self.n
+        if self.bnorm <= 1e-30:
+            L.call("tg_fill64", n, L.ptr(self.u), 0.0, st)
+            self.last = SolveStats(0, 0, True, 0.0)
+            return self.last
+        if not self.warm_start:
+            L.call("tg_fill64", n, L.ptr(self.u), 0.0, st)
+        its, cycles, conv = self._fgmres_run()
+        if not conv and self.warm_start:
+            L.call("tg_fill64", n, L.ptr(self.u), 0.0, st)
+            its, c2, conv = self._fgmres_run()
+            cycles += c2
+        rnorm = self.residual64()
+        rel = rnorm / np.maximum(self.bnorms, 1e-300)
+        self.last = SolveStats(its, cycles + 1, bool(conv), float(rel.max()))
+        return self.last
+
+    def _fgmres_run(self) -> tuple[int, int, bool]:
+        st, n, m, tol = self.stream, self.n, self.restart, self.tol
+        if self._krylov is None or self._krylov[0].shape[0] != m + 1:
+            f64 = dict(dtype=torch.float64, device=self.dev)
+            self._krylov = (torch.zeros(m + 1, n, **f64), torch.zeros(m, n, **f64), torch.zeros(n, **f64),
+                            torch.zeros(m + 2, **f64), torch.zeros(m, **f64))
+        V, Z, w, h, coef = self._krylov
+        b_norm = self.bnorm
+        total = cycles = 0
+        converged = False
+        while total < self.max_krylov:
+            beta = float(self.residual64()[0])            # r64 = F - K u, s[4] = r.r
+            cycles += 1
+            if not np.isfinite(beta):
+                raise FloatingPointError("non-finite residual in FGMRES")
+            if beta / b_norm <= tol:
+                converged = True
+                break
+            L.call("tg_normalize64", n, L.ptr(self.r64), L.ptr(self.s[4]), L.ptr(V[0]), L.ptr(self.r), st)
+            H = np.zeros((m + 1, m))
+            cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
+            g[0] = beta
+            j_final = 0
+            for j in range(min(m, self.max_krylov - total)):
+                z = self.gmg.apply(self.r, st)             # M V[j] (FP32 input, the copy of V[j])
+                L.call("tg_axpy64", n, L.ptr(Z[j]), None, L.ptr(z), None, 1.0, 0, st)
+                L.call("tg_apply64_b", self.grid, L.ptr(self.mask), L.ptr(self.E64), 1, L.ptr(Z[j]), None,
+                       L.ptr(w), None, None, st)
+                for i in range(j + 1):                     # modified Gram-Schmidt
+                    L.call("tg_dot64_b", n, 1, L.ptr(V[i]), L.ptr(w), L.ptr(h[i:i + 1]), st)
+                    L.call("tg_axpy64", n, L.ptr(w), L.ptr(V[i]), None, L.ptr(h[i:i + 1]), -1.0, 1, st)
+                L.call("tg_dot64_b", n, 1, L.ptr(w), L.ptr(w), L.ptr(h[j + 1:j + 2]), st)
+                L.call("tg_normalize64", n, L.ptr(w), L.ptr(h[j + 1:j + 2]), L.ptr(V[j + 1]), L.ptr(self.r), st)
+                col = h[:j + 2].cpu().numpy()
+                H[:j + 1, j] = col[:j + 1]
+                h_norm = float(np.sqrt(max(col[j + 1], 0.0)))
+                H[j + 1, j] = h_norm
+                for i in range(j):                         # previous Givens rotations
+                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                    H[i, j] = t
+                den = np.sqrt(H[j, j] ** 2 + H[j + 1, j] ** 2)
+                cs[j], sn[j] = (1.0, 0.0) if den < 1e-300 else (H[j, j] / den, H[j + 1, j] / den)
+                H[j, j] = cs[j] * H[j, j] + sn[j] * H[j + 1, j]
+                H[j + 1, j] = 0.0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j] * g[j]
+                total += 1
+                j_final = j + 1
+                if not np.isfinite(g[j + 1]):
+                    raise FloatingPointError("non-finite FGMRES residual estimate")
+                if abs(g[j + 1]) / b_norm <= tol or h_norm <= 1e-14:
+                    converged = True
+                    break
+            if j_final:
+                R = np.triu(H[:j_final, :j_final])
+                try:
+                    y = np.linalg.solve(R, g[:j_final])
+                except np.linalg.LinAlgError:
+                    y = np.linalg.lstsq(R, g[:j_final], rcond=None)[0]
+                coef[:j_final].copy_(torch.from_numpy(np.ascontiguousarray(y, dtype=np.float64)))
+                for k in range(j_final):                   # u += sum_k y_k Z[k]
+                    L.call("tg_axpy64", n, L.ptr(self.u), L.ptr(Z[k]), None, L.ptr(coef[k:k + 1]), 1.0, 1, st)
+            if converged:
+                break
+        return total, cycles, converged
+
     def _read_rr(self, st, speculate: bool) -> np.ndarray:
         """r.r of the last iteration on the host. With speculate, the next
         V-cycle's first part (gmg.begin, which touches only the hierarchy's

@@ -109,7 +109,9 @@ class SIMPSolver:
     run that finds them there resumes from them, as run_simp does,
     pub_simp_solver.py:495-516, 597-603), callback (run_simp's controller,
     topogpu_amd.control), min_penal_for_best, max_gray_for_best,
-    tail_default_iters, beta_max, seed.
+    tail_default_iters, beta_max, seed, outer_solver ("pcg" | "fgmres": the
+    state solve's outer Krylov method, SolverV4's gmg_outer_solver) and
+    restart (FGMRES restart length, gmg_restart).
     """
 
     def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
@@ -199,7 +201,8 @@ class SIMPSolver:
                 "iteration": it, "compliance": float(step.compliance), "outer_iters": int(step.iters),
                 "rho_mean": float(rho_phys.mean()), "rho_min": float(rho_phys.min()),
                 "rho_max": float(rho_phys.max()), "grayness": gray, "wall_s": wall,
-                "outer_solver": "pcg", "change": float(step.change), "ir_restarts": int(step.restarts),
+                "outer_solver": engine.fea.outer_solver, "change": float(step.change),
+                "ir_restarts": int(step.restarts),
                 "rel_residual": float(step.rel_residual), "converged": bool(step.converged),
             })
             params_log.append({"iter": it, "phase": phase, "penal": ctl.penal, "rmin": ctl.rmin, "move": ctl.move,
@@ -317,7 +320,7 @@ class SIMPSolver:
             history.append({
                 "iteration": iteration, "compliance": float(step.compliance), "outer_iters": int(step.iters),
                 "rho_mean": float(rho.mean()), "rho_min": float(rho.min()), "rho_max": float(rho.max()),
-                "grayness": grayness(rho), "wall_s": wall, "outer_solver": "pcg",
+                "grayness": grayness(rho), "wall_s": wall, "outer_solver": engine.fea.outer_solver,
                 "change": float(step.change), "ir_restarts": int(step.restarts),
                 "rel_residual": float(step.rel_residual), "converged": bool(step.converged),
             })
