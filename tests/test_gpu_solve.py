@@ -273,3 +273,26 @@ def test_simp_c0_with_fgmres_matches_reference_golden(gpu, golden):
     assert all(r["outer_solver"] == "fgmres" for r in res.history)
     assert np.allclose(comp, g["compliance_history"], rtol=1e-6, atol=0), comp / g["compliance_history"] - 1
     assert np.abs(res.rho_final - g["rho_final"]).max() < 1e-5
+
+
+def test_probe_quality_matches_host(gpu):
+    """probe_quality (multigrid_v4.py:821-832): the diagnostics after a solve
+    equal the same quantities computed on the host from the V-cycle's output
+    and the level-0 inverse diagonal."""
+    dims = (16, 8, 8)
+    free, F, ndof = cantilever_bc(dims, point_support=True)
+    rng = np.random.default_rng(23)
+    E = 1e-9 + rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3
+    s = HipStateSolver(dims, free, F, n_levels=3, tol=1e-8, probe_quality=True)
+    E64 = torch.from_numpy(E).to(DEV)
+    s.set_modulus(E64, E64.float(), float(E.max()))
+    assert s.solve().converged
+    q = s.last_multigrid_quality
+    F32 = h(s.F).astype(np.float32)
+    s.r.copy_(torch.from_numpy(F32).to(DEV))
+    z = h(s.gmg.apply(s.r, s.stream)).astype(np.float64)
+    jac = (h(s.gmg.levels[0].dinv).astype(np.float32) * F32).astype(np.float64)
+    assert q["finite"] and q["pd"] == bool(F32.astype(np.float64) @ z > 0.0) and q["pd"]
+    assert q["z_norm"] == pytest.approx(np.linalg.norm(z), rel=1e-10)
+    assert q["jacobi_norm"] == pytest.approx(np.linalg.norm(jac), rel=1e-10)
+    assert q["z_over_jacobi"] == pytest.approx(np.linalg.norm(z) / np.linalg.norm(jac), rel=1e-10)

@@ -85,7 +85,7 @@ class HipStateSolver:
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
                  smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", outer_solver: str = "pcg",
-                 restart: int = 32, device=None):
+                 restart: int = 32, probe_quality: bool = False, device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if precision not in PRECISIONS:
@@ -115,6 +115,9 @@ class HipStateSolver:
             raise ValueError("restart must be >= 1")
         self.outer_solver, self.restart = outer_solver, int(restart)
         self._krylov = None
+        # one extra V-cycle after each solve for its diagnostics
+        self.probe = bool(probe_quality)
+        self.last_multigrid_quality: dict | None = None
         F[:, np.setdiff1d(np.arange(self.n), free)] = 0.0
         self.F = torch.from_numpy(F.ravel()).to(self.dev)
         self.bnorms = np.linalg.norm(F, axis=1)
@@ -176,8 +179,37 @@ class HipStateSolver:
         return np.sqrt(np.maximum(self.s[4].cpu().numpy(), 0.0))
 
     def solve(self) -> SolveStats:
-        if self.outer_solver == "fgmres":
-            return self._solve_fgmres()
+        stats = self._solve_fgmres() if self.outer_solver == "fgmres" else self._solve_pcg()
+        if self.probe:
+            self.last_multigrid_quality = self.probe_quality()
+        return stats
+
+    def probe_quality(self) -> dict:
+        """Diagnostics of the V-cycle as a preconditioner on F (probe_quality,
+        multigrid_v4.py:821-832, called after every solve by solver_v4.py:
+        232): z = M F finite, F.z > 0, ||z||, ||D^-1 F|| and their ratio.
+        F enters the V-cycle in FP32 as the PCG residual does; the sums are
+        FP64. With several load cases every value is a list, one per case."""
+        st, n, nr = self.stream, self.n, self.nrhs
+        if not hasattr(self, "_probe_s"):
+            self._probe_s = torch.zeros(3, nr, dtype=torch.float64, device=self.dev)
+            self._one = torch.ones(1, dtype=torch.float64, device=self.dev)
+        ps = self._probe_s
+        L.call("tg_normalize64", nr * n, L.ptr(self.F), L.ptr(self._one), None, L.ptr(self.r), st)   # r = float(F)
+        z = self.gmg.apply(self.r, st)
+        L.call("tg_dot32_b", n, nr, L.ptr(z), L.ptr(self.r), L.ptr(z), L.ptr(ps[0]), L.ptr(ps[1]), st)
+        L.call("tg_cheb_init32_b", n, nr, L.ptr(self.r), L.ptr(self.gmg.levels[0].dinv), 1.0, L.ptr(self.p),
+               L.ptr(self.q), None, st)                                                      # p = D^-1 F
+        L.call("tg_dot32_b", n, nr, L.ptr(self.p), L.ptr(self.p), None, L.ptr(ps[2]), None, st)
+        fz, zz, jj = ps.cpu().numpy()
+        zn, jn = np.sqrt(np.maximum(zz, 0.0)), np.sqrt(np.maximum(jj, 0.0))
+        q = {"finite": np.isfinite(zz), "pd": fz > 0.0, "z_norm": zn, "jacobi_norm": jn,
+             "z_over_jacobi": zn / np.maximum(jn, 1e-30)}
+        if nr == 1:
+            return {k: (bool(v[0]) if v.dtype == bool else float(v[0])) for k, v in q.items()}
+        return {k: [bool(x) if v.dtype == bool else float(x) for x in v] for k, v in q.items()}
+
+    def _solve_pcg(self) -> SolveStats:
         st, n, nr = self.stream, self.n, self.nrhs
         if not self.warm_start:
             L.call("tg_fill64", nr * n, L.ptr(self.u), 0.0, st)
