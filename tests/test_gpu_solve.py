@@ -296,3 +296,29 @@ def test_probe_quality_matches_host(gpu):
     assert q["z_norm"] == pytest.approx(np.linalg.norm(z), rel=1e-10)
     assert q["jacobi_norm"] == pytest.approx(np.linalg.norm(jac), rel=1e-10)
     assert q["z_over_jacobi"] == pytest.approx(np.linalg.norm(z) / np.linalg.norm(jac), rel=1e-10)
+
+
+def test_warm_start_policy(gpu, monkeypatch):
+    """solver_v4.py:162-164, 192-209: a penalty jump above 0.5 drops the warm
+    start; a warm-started solve that does not converge is retried from zero."""
+    from topogpu_amd.bc import generate_bc
+    from topogpu_amd.hip.simp import HipSIMP
+
+    prob = tg.unit_cantilever((16, 8, 4), volfrac=0.30, rmin=1.5)
+    bc = generate_bc(prob.spec)
+    eng = HipSIMP(prob.nel, bc.free_dofs, bc.F, volfrac=0.3, rmin=1.5, move=0.2, tol=1e-8, max_krylov=400)
+    resets, runs = [], []
+    reset, run = eng.fea.reset_warm_start, eng.fea._pcg_run
+    monkeypatch.setattr(eng.fea, "reset_warm_start", lambda: (resets.append(1), reset())[1])
+    monkeypatch.setattr(eng.fea, "_pcg_run", lambda zero: (runs.append(zero), run(zero))[1])
+    eng.step()
+    eng.penal = 3.5
+    eng.step()
+    assert resets == [] and runs == [False, False]
+    eng.penal = 4.1
+    eng.step()
+    assert resets == [1] and eng.fea.last.converged
+    eng.fea.max_krylov = 1                    # too small a budget: the warm start fails, then zero does
+    runs.clear()
+    eng.step()
+    assert runs == [False, True] and not eng.fea.last.converged

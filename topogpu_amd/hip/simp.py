@@ -49,6 +49,7 @@ class HipSIMP:
         self.n_elem = int(np.prod(self.dims))
         self.volfrac, self.rmin, self.penal, self.move = float(volfrac), float(rmin), float(penal), float(move)
         self.use_heaviside, self.beta, self.eta = bool(use_heaviside), float(beta), float(eta)
+        self._prev_penal: float | None = None
         self.fea = HipStateSolver(self.dims, free, F, **solver_kw)
         self.dev = self.fea.dev
         # multi-load: the objective is sum_k w_k F_k.u_k (equal weights of 1
@@ -128,7 +129,12 @@ class HipSIMP:
                L.ptr(self.rho_phys), L.ptr(self.dh), L.ptr(self.E64), L.ptr(self.E32), self.stream)
 
     def state(self):
-        """Set moduli, build the hierarchy and solve; returns (compliance, stats)."""
+        """Set moduli, build the hierarchy and solve; returns (compliance,
+        stats). A penalty change above 0.5 since the last solve drops the
+        warm start (solver_v4.py:162-164)."""
+        if self._prev_penal is not None and abs(self.penal - self._prev_penal) > 0.5:
+            self.fea.reset_warm_start()
+        self._prev_penal = self.penal
         emax = float(self.E64.max().item())
         self.fea.set_modulus(self.E64, self.E32, emax)
         stats = self.fea.solve()

@@ -209,9 +209,25 @@ class HipStateSolver:
             return {k: (bool(v[0]) if v.dtype == bool else float(v[0])) for k, v in q.items()}
         return {k: [bool(x) if v.dtype == bool else float(x) for x in v] for k, v in q.items()}
 
+    def reset_warm_start(self) -> None:
+        """Start the next solve from zero (solver_v4.py:162-164 drops the
+        previous solution when the penalty jumps)."""
+        L.call("tg_fill64", self.nrhs * self.n, L.ptr(self.u), 0.0, self.stream)
+
     def _solve_pcg(self) -> SolveStats:
+        """Flexible PCG in FP32 inside FP64 residual restarts; a warm start
+        that does not converge within the budget is retried once from zero
+        (solver_v4.py:192-209)."""
+        stats = self._pcg_run(zero=not self.warm_start)
+        if not stats.converged and self.warm_start:
+            retry = self._pcg_run(zero=True)
+            self.last = SolveStats(retry.iters, stats.restarts + retry.restarts, retry.converged,
+                                   retry.rel_residual)
+        return self.last
+
+    def _pcg_run(self, zero: bool) -> SolveStats:
         st, n, nr = self.stream, self.n, self.nrhs
-        if not self.warm_start:
+        if zero:
             L.call("tg_fill64", nr * n, L.ptr(self.u), 0.0, st)
         target = self.tol * self.bnorms
         its = restarts = 0
@@ -220,8 +236,9 @@ class HipStateSolver:
         if not np.all(np.isfinite(rnorm)):
             raise FloatingPointError("non-finite residual at the start of the state solve")
         far = rnorm > 10.0 * self.bnorms
-        if self.warm_start and far.any():
-            # a warm start that is far off is worse than zero (solver_v4.py:201-209)
+        if not zero and far.any():
+            # a warm start whose residual exceeds ten times ||F|| is worse
+            # than zero: drop it before spending iterations on it
             for k in np.flatnonzero(far):
                 L.call("tg_fill64", n, L.ptr(self._slice(self.u, k)), 0.0, st)
             rnorm = self.residual64()
