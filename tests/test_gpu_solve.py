@@ -322,3 +322,34 @@ def test_warm_start_policy(gpu, monkeypatch):
     runs.clear()
     eng.step()
     assert runs == [False, True] and not eng.fea.last.converged
+
+
+@pytest.mark.parametrize("schedule", [["fp32", "bf16", "bf16"], ["bf16", "fp64"]])
+def test_level_precisions_schedule(gpu, schedule):
+    """level_precisions (multigrid_v4.py:673-682): level 0 takes the first
+    entry, the coarse levels the rest (missing ones the most accurate kind);
+    every schedule still solves to the direct solution."""
+    from topogpu_amd.hip import lib as L
+
+    dims = (24, 12, 6)
+    # the face clamp keeps level 1 a q8 level; the point supports make it a
+    # stencil level (its free mask is not the injection of the fine one)
+    free, F, ndof = cantilever_bc(dims, point_support=schedule[0] == "bf16")
+    rng = np.random.default_rng(24)
+    E = 1e-9 + rng.uniform(1e-3, 1.0, int(np.prod(dims))) ** 3
+    U_ref = oracle.solve_state(oracle.edof_table_3d(*dims), E, free, F, ndof)
+    s = HipStateSolver(dims, free, F, n_levels=4, tol=1e-9, max_krylov=400, level_precisions=schedule)
+    assert s.precision == schedule[0]
+    bf16 = [len(schedule) > k and schedule[k] == "bf16" for k in range(1, 3)]
+    if s.gmg.l1_q8:
+        assert s.gmg.levels[1].kind == (L.TG_OP_Q8_BF16 if bf16[0] else L.TG_OP_Q8_F32)
+    else:
+        assert s.gmg.levels[1].kind == (L.TG_OP_STENCIL_BF16 if bf16[0] else L.TG_OP_STENCIL_F32)
+    assert s.gmg.levels[2].kind == (L.TG_OP_STENCIL_BF16 if bf16[1] else L.TG_OP_STENCIL_F32)
+    E64 = torch.from_numpy(E).to(DEV)
+    s.set_modulus(E64, E64.float(), float(E.max()))
+    st = s.solve()
+    assert st.converged and st.rel_residual <= 1e-9
+    assert rel_l2(h(s.u), U_ref) < 1e-6
+    with pytest.raises(ValueError):
+        HipStateSolver(dims, free, F, n_levels=4, level_precisions=["bf16", "fp16"])

@@ -77,7 +77,8 @@ class GMG:
                  l1_q8_accurate_op: int | None = None, fine_degree: int = 2, coarse_degree=2,
                  cheb_lower_frac: float = 1.0 / 30.0, cheb_safety: float = 1.1, device: torch.device,
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
-                 nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32, smoother: str = "chebyshev",
+                 nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32, coarse_precisions=None,
+                 smoother: str = "chebyshev",
                  omega: float = 0.5, cycle: str = "v"):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
@@ -151,6 +152,22 @@ class GMG:
             l1.kind = l1_q8_op
             l1.kind_accurate = l1_q8_accurate_op if l1_q8_accurate_op is not None else l1_q8_op
         self.l1_q8 = l1.kind in (L.TG_OP_Q8_F32, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16X3)
+        if coarse_precisions is not None:
+            # a per-level schedule for levels 1.. (level_precisions[1:],
+            # multigrid_v4.py:673-682; missing entries are the most accurate
+            # kind, as the reference pads with "fp64"): "bf16" selects BF16
+            # operands, "bf16x3" the split BF16 level-1 product, "fp32" and
+            # "fp64" FP32 operands (no level smooths in FP64 here)
+            for lv in range(1, n_levels):
+                pr = coarse_precisions[lv - 1] if lv - 1 < len(coarse_precisions) else "fp64"
+                if pr not in ("bf16", "bf16x3", "fp32", "fp64"):
+                    raise ValueError(f"level precision must be bf16, bf16x3, fp32 or fp64; got {pr!r}")
+                Lv = self.levels[lv]
+                if lv == 1 and self.l1_q8:
+                    Lv.kind = {"bf16": L.TG_OP_Q8_BF16, "bf16x3": L.TG_OP_Q8_BF16X3}.get(pr, L.TG_OP_Q8_F32)
+                else:
+                    Lv.kind = L.TG_OP_STENCIL_BF16 if pr == "bf16" else L.TG_OP_STENCIL_F32
+                Lv.kind_accurate = Lv.kind
         # Galerkin build plan: aligned levels come straight from the fine E by
         # one gather + DGEMM (levels 1 and 2); the rest from the level above.
         even = lambda d: all(n % 2 == 0 for n in d)   # noqa: E731

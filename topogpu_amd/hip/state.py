@@ -85,9 +85,16 @@ class HipStateSolver:
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
                  smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", outer_solver: str = "pcg",
-                 restart: int = 32, probe_quality: bool = False, device=None):
+                 restart: int = 32, probe_quality: bool = False, level_precisions=None, device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
+        if level_precisions is not None:
+            # level 0's entry is the fine-level precision; the rest go to the
+            # coarse levels (GalerkinMatFreeGMG level_precisions)
+            level_precisions = list(level_precisions)
+            if not level_precisions:
+                raise ValueError("level_precisions must name at least level 0")
+            precision = level_precisions[0]
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}; got {precision!r}")
         self.dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
@@ -131,7 +138,8 @@ class HipStateSolver:
                        l1_q8_op=ops[4], l1_q8_accurate_op=ops[3], fine_degree=fine_degree,
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
                        level1=level1, nrhs=self.nrhs, coarse_stencil=ops[5], smoother=smoother, omega=omega,
-                       cycle=cycle, device=self.dev)
+                       cycle=cycle, coarse_precisions=None if level_precisions is None else level_precisions[1:],
+                       device=self.dev)
         nb = self.nrhs * self.n
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.u = torch.zeros(nb, dtype=torch.float64, device=self.dev)
