@@ -306,6 +306,11 @@ int tg_stencil64(const tg_grid* g, int32_t m, const int32_t* offs_host, const do
                  const double* in, const double* in_scale, const double* out_scale,
                  double* out, void* stream);
 
+/* Passive elements: rho[e] = rho_min where mask[e] == 1 (void), 1 where
+ * mask[e] == 2 (solid), unchanged where 0 (apply_passive_mask,
+ * bc_generator.py:454-470, applied after each OC update when asked). */
+int tg_passive64(int64_t n, const int32_t* mask, double* rho, double rho_min, void* stream);
+
 /* rho_phys = heaviside(rho_f, beta, eta) (identity unless beta > 1e-6),
  * dh = d rho_phys / d rho_f, E = Emin + (E0-Emin) rho_phys^penal (FP64 and
  * FP32). pub_simp_solver.py:245-253, 347-350. Any output may be NULL. */

@@ -150,3 +150,31 @@ def test_checkpoint_files_roundtrip(tmp_path):
     assert list(got["compliance_hist"]) == [3.0, 2.5] and got["history"][1] == {"iteration": 2}
     (tmp_path / "ckpt_4x3x2_case.json").unlink()
     assert load_checkpoint(path) is None
+
+
+def test_passive_mask_matches_bruteforce():
+    """build_passive_mask (bc_generator.py:84-93, 318-358) restated as a
+    loop over elements: centroids (i + 1/2) h, a cylinder along z and an x-y
+    box with inclusive boundaries, later regions overwriting earlier ones."""
+    from topogpu_amd.problem_spec import CircularRegion, RectangularRegion
+
+    spec = tg.ProblemSpec(Lx=8.0, Ly=4.0, Lz=2.0, nelx=8, nely=4, nelz=2, volfrac=0.4,
+                          supports=[tg.EdgeSupport("left", "fixed")],
+                          loads=[tg.PointLoad(x=8.0, y=0.0, z=1.0, fy=-1.0)],
+                          passive_regions=[RectangularRegion(0.5, 0.5, 2.5, 1.5, "solid"),
+                                           CircularRegion(2.5, 1.5, 1.0, "void")])
+    got = bc.generate_bc(spec).passive_mask
+    want = np.zeros(64, dtype=np.int32)
+    for i in range(8):
+        for j in range(4):
+            for k in range(2):
+                x, y = i + 0.5, j + 0.5
+                v = 0
+                if 0.5 <= x <= 2.5 and 0.5 <= y <= 1.5:
+                    v = 2
+                if ((x - 2.5) ** 2 + (y - 1.5) ** 2) ** 0.5 <= 1.0:
+                    v = 1
+                want[(i * 4 + j) * 2 + k] = v
+    assert np.array_equal(got, want) and (got == 1).sum() == 10 and (got == 2).sum() == 6
+    spec.passive_regions = [CircularRegion(100.0, 100.0, 1.0, "void")]
+    assert bc.generate_bc(spec).passive_mask is None

@@ -353,3 +353,32 @@ def test_level_precisions_schedule(gpu, schedule):
     assert rel_l2(h(s.u), U_ref) < 1e-6
     with pytest.raises(ValueError):
         HipStateSolver(dims, free, F, n_levels=4, level_precisions=["bf16", "fp16"])
+
+
+def test_passive_regions_enforced_when_asked(gpu):
+    """apply_passive=True enforces the passive mask after each OC update
+    (apply_passive_mask, bc_generator.py:454-470); by default the mask is
+    computed but, as in the reference loops, not applied."""
+    from topogpu_amd.problem_spec import CircularRegion, RectangularRegion
+
+    spec = tg.ProblemSpec(Lx=16.0, Ly=8.0, Lz=4.0, nelx=16, nely=8, nelz=4, volfrac=0.3,
+                          supports=[tg.EdgeSupport("left", "fixed")],
+                          loads=[tg.PointLoad(x=16.0, y=4.0, z=0.0, fz=-1.0)],
+                          passive_regions=[CircularRegion(8.0, 4.0, 2.0, "void"),
+                                           RectangularRegion(12.0, 0.0, 16.0, 2.0, "solid")])
+    from topogpu_amd.bc import generate_bc
+    from topogpu_amd.hip.simp import HipSIMP
+
+    prob = tg.TopologyProblem("passive", spec, filter_radius=1.5)
+    b = generate_bc(spec)
+    mask = b.passive_mask
+    assert (mask == 1).any() and (mask == 2).any()
+    kw = dict(backend="hip", tol=1e-8, max_iter=3, move=0.2)
+    res = tg.SIMPSolver(apply_passive=True, **kw).solve(prob)
+    assert res.summary["n_iter"] == 3
+    off = tg.SIMPSolver(**kw).solve(prob)
+    eng = HipSIMP(prob.nel, b.free_dofs, b.F, volfrac=0.3, rmin=1.5, move=0.2, tol=1e-8, passive=b.passive_mask)
+    eng.step()
+    rho = eng.rho.cpu().numpy()
+    assert np.all(rho[mask == 1] == 1e-3) and np.all(rho[mask == 2] == 1.0)
+    assert not np.array_equal(res.rho_final, off.rho_final)

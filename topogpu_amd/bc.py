@@ -13,7 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .problem_spec import DistributedLoad, EdgeSupport, PointLoad, PointSupport
+from .problem_spec import (CircularRegion, DistributedLoad, EdgeSupport, PointLoad, PointSupport,
+                           RectangularRegion)
 
 # face -> (normal axis, index on that axis is 0 or n, outward sign)
 _FACES = {"left": (0, 0, -1.0), "right": (0, 1, 1.0), "bottom": (1, 0, -1.0), "top": (1, 1, 1.0),
@@ -26,6 +27,32 @@ class BoundaryConditions:
     free_dofs: np.ndarray
     F: np.ndarray
     ndof: int
+    passive_mask: np.ndarray | None = None   # per element: 0 free, 1 void, 2 solid
+
+
+def passive_mask(spec) -> np.ndarray | None:
+    """Passive elements by centroid (build_passive_mask, bc_generator.py:
+    84-93, 318-358): a circular region is a cylinder along z (distance in x-y
+    at most the radius), a rectangular one an x-y box prism, boundaries
+    inclusive; later regions overwrite earlier ones. 1 = void, 2 = solid;
+    None when no element is passive."""
+    if not spec.passive_regions:
+        return None
+    (nx, ny, nz), (lx, ly, _) = _grid_axes(spec)
+    hx, hy = lx / nx, ly / ny
+    cx = np.repeat(np.arange(nx) * hx + 0.5 * hx, ny * nz)          # element e = (i*ny + j)*nz + k
+    cy = np.tile(np.repeat(np.arange(ny) * hy + 0.5 * hy, nz), nx)
+    mask = np.zeros(nx * ny * nz, dtype=np.int32)
+    for r in spec.passive_regions:
+        if isinstance(r, CircularRegion):
+            dx, dy = cx - r.cx, cy - r.cy
+            inside = np.sqrt(dx * dx + dy * dy) <= r.radius
+        elif isinstance(r, RectangularRegion):
+            inside = (cx >= r.x0) & (cx <= r.x1) & (cy >= r.y0) & (cy <= r.y1)
+        else:
+            raise TypeError(f"unsupported passive region {r!r}")
+        mask[inside] = 1 if r.kind == "void" else 2
+    return mask if np.any(mask) else None
 
 
 def _grid_axes(spec):
@@ -63,8 +90,6 @@ def generate_bc(spec) -> BoundaryConditions:
         raise ValueError("ProblemSpec validation failed:\n" + "\n".join(f"  - {e}" for e in errors))
     if not spec.is_3d:
         raise ValueError("the hip backend requires a 3D problem (Lz > 0 and nelz > 0)")
-    if spec.passive_regions:
-        raise NotImplementedError("passive regions are not supported by the hip backend")
     n, L = _grid_axes(spec)
     ndof = 3 * (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
     fixed = np.zeros(ndof, dtype=bool)
@@ -101,4 +126,4 @@ def generate_bc(spec) -> BoundaryConditions:
     if free.size == 0 or np.max(np.abs(F[free])) < 1e-30:
         raise ValueError("Force vector is zero on all free DOFs. "
                          "Check that loads are not placed on fully-fixed supports.")
-    return BoundaryConditions(np.nonzero(fixed)[0], free, F, ndof)
+    return BoundaryConditions(np.nonzero(fixed)[0], free, F, ndof, passive_mask(spec))

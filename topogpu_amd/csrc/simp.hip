@@ -880,6 +880,24 @@ extern "C" int tg_stencil64(const tg_grid* gp, int32_t m, const int32_t* offs, c
   return TG_OK;
 }
 
+// Passive elements after the OC update (apply_passive_mask,
+// bc_generator.py:454-470): void elements to rho_min, solid ones to 1.
+__global__ void k_passive(long long n, const int32_t* __restrict__ mask, double* __restrict__ rho, double rho_min) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int m = mask[e];
+  if (m == 1) rho[e] = rho_min;
+  else if (m == 2) rho[e] = 1.0;
+}
+
+extern "C" int tg_passive64(int64_t n, const int32_t* mask, double* rho, double rho_min, void* stream) {
+  TG_REQUIRE(n > 0 && mask && rho, "passive64: bad argument");
+  hipLaunchKernelGGL(k_passive, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                     mask, rho, rho_min);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+
 extern "C" int tg_density_chain(int64_t n, const double* rf, double beta, double eta, double penal, double emin,
                                 double e0, double* rp, double* dh, double* E64, float* E32, void* s) {
   TG_REQUIRE(n > 0 && rf, "density_chain: bad argument");

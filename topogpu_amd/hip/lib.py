@@ -76,6 +76,7 @@ _SIGS = {
     "tg_axpy_f32_to_f64": [_i64, _vp, _vp, _vp],
     "tg_axpy64": [_i64, _vp, _vp, _vp, _vp, _f64, _i32, _vp],
     "tg_normalize64": [_i64, _vp, _vp, _vp, _vp, _vp],
+    "tg_passive64": [_i64, _vp, _vp, _f64, _vp],
     "tg_cheb_init32": [_i64, _vp, _vp, _f32, _vp, _vp, _vp, _vp],
     "tg_power_step32": [_i64, _vp, _vp, _vp, _vp, _vp],
     "tg_fill32": [_i64, _vp, _f32, _vp],

@@ -41,7 +41,7 @@ class HipSIMP:
     def __init__(self, dims, free, F, *, volfrac: float, rmin: float = 1.5, penal: float = 3.0,
                  move: float = 0.2, use_heaviside: bool = True, beta: float = 1.0, eta: float = 0.5,
                  rho0: np.ndarray | None = None, weights=None, formulation: str = "filtered",
-                 rho_min: float = 1.0e-3, **solver_kw):
+                 rho_min: float = 1.0e-3, passive: np.ndarray | None = None, **solver_kw):
         if formulation not in ("filtered", "solverv4"):
             raise ValueError(f"formulation must be 'filtered' or 'solverv4'; got {formulation!r}")
         self.formulation, self.rho_min = formulation, float(rho_min)
@@ -50,8 +50,16 @@ class HipSIMP:
         self.volfrac, self.rmin, self.penal, self.move = float(volfrac), float(rmin), float(penal), float(move)
         self.use_heaviside, self.beta, self.eta = bool(use_heaviside), float(beta), float(eta)
         self._prev_penal: float | None = None
+        # passive elements (0 free, 1 void, 2 solid) enforced after each OC
+        # update; None: every element is designable
+        pm = None
+        if passive is not None:
+            pm = np.ascontiguousarray(passive, dtype=np.int32).ravel()
+            if pm.size != self.n_elem or not np.isin(pm, (0, 1, 2)).all():
+                raise ValueError("passive must hold 0, 1 or 2 per element")
         self.fea = HipStateSolver(self.dims, free, F, **solver_kw)
         self.dev = self.fea.dev
+        self.passive = None if pm is None else torch.from_numpy(pm).to(self.dev)
         # multi-load: the objective is sum_k w_k F_k.u_k (equal weights of 1
         # by default, so it is the sum of the single-load compliances)
         w = np.ones(self.fea.nrhs) if weights is None else np.asarray(weights, dtype=np.float64).ravel()
@@ -169,6 +177,7 @@ class HipSIMP:
                 break
         L.call("tg_oc_mean_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.move, self.rho_min,
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
+        self._apply_passive(self.rho_new)
         change = float(self.scal[0].item())
         n_steps = int(self.oc_state[3].item())
         self.rho, self.rho_new = self.rho_new, self.rho
@@ -198,10 +207,19 @@ class HipSIMP:
                 break
         L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
+        self._apply_passive(self.rho_new)
         change = float(self.scal[0].item())
         n_steps = int(self.oc_state[4].item())
         self.rho, self.rho_new = self.rho_new, self.rho
         return change, n_steps
+
+    def _apply_passive(self, rho) -> None:
+        """Void elements to the lower density bound, solid ones to 1
+        (apply_passive_mask, bc_generator.py:454-470; rho_min = 1e-3 there,
+        the filtered OC's bound)."""
+        if self.passive is not None:
+            lo = self.rho_min if self.formulation == "solverv4" else 1.0e-3
+            L.call("tg_passive64", self.n_elem, L.ptr(self.passive), L.ptr(rho), lo, self.stream)
 
     def step(self) -> StepResult:
         self.physical()

@@ -111,7 +111,10 @@ class SIMPSolver:
     topogpu_amd.control), min_penal_for_best, max_gray_for_best,
     tail_default_iters, beta_max, seed, outer_solver ("pcg" | "fgmres": the
     state solve's outer Krylov method, SolverV4's gmg_outer_solver) and
-    restart (FGMRES restart length, gmg_restart).
+    restart (FGMRES restart length, gmg_restart), apply_passive (enforce the
+    spec's passive regions after each OC update, apply_passive_mask,
+    bc_generator.py:454-470; off by default, as the reference loops leave
+    them unapplied).
     """
 
     def __init__(self, backend: str = "hip", linear_solver: str = "auto", optimizer: str = "oc",
@@ -169,8 +172,9 @@ class SIMPSolver:
         bc = generate_bc(spec)
         force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
         rmin = problem.filter_radius or spec.rmin or 1.5
+        passive = bc.passive_mask if opts.pop("apply_passive", False) else None
         engine = HipSIMP(problem.nel, bc.free_dofs, force, volfrac=spec.volfrac, rmin=rmin, move=self.move,
-                         tol=self.tol, max_krylov=self.max_krylov, **opts)
+                         tol=self.tol, max_krylov=self.max_krylov, passive=passive, **opts)
         nx, ny, nz = problem.nel
         params = C.RunParams(nelx=nx, nely=ny, nelz=nz, volfrac=spec.volfrac, penal=engine.penal, rmin=rmin,
                              move=self.move, max_iter=self.max_iter, tol=conv_tol, compliance_tol=compliance_tol,
@@ -308,8 +312,9 @@ class SIMPSolver:
         bc = generate_bc(spec)
         force = bc.F if problem.force_override is None else np.asarray(problem.force_override, dtype=np.float64)
         rmin = problem.filter_radius or spec.rmin or 1.5   # unused by the formulation; kept for the engine
+        passive = bc.passive_mask if opts.pop("apply_passive", False) else None
         engine = HipSIMP(problem.nel, bc.free_dofs, force, volfrac=spec.volfrac, rmin=rmin, move=self.move,
-                         rho_min=self.rho_min, tol=self.tol, max_krylov=self.max_krylov, **opts)
+                         rho_min=self.rho_min, tol=self.tol, max_krylov=self.max_krylov, passive=passive, **opts)
         history: list[dict[str, Any]] = []
         t_total = time.perf_counter()
         for iteration in range(1, self.max_iter + 1):
