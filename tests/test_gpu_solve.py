@@ -216,7 +216,7 @@ def test_checkpoint_resume_equals_uninterrupted_run(gpu, tmp_path, controller):
 
 
 @pytest.mark.parametrize("case", ["A", "B"])
-def test_controlled_run_matches_reference_golden(gpu, golden, case):
+def test_controlled_run_matches_reference_golden(gpu, golden, case, tmp_path):
     """run_simp with a controller (pub_simp_solver.py:412-657) on the GPU:
     actions, best tracking, restarts, stop and tail against the reference's
     own run with the same controller (16x8x4 cantilever)."""
@@ -233,6 +233,7 @@ def test_controlled_run_matches_reference_golden(gpu, golden, case):
     assert [r["iteration"] for r in res.history] == list(range(1, s["n_iter"] + 1))
     check_controlled_run(g, comp, res.rho_final, s["final_compliance"], s["n_iter"], dict(s, best_rho=res.best_rho),
                          s["params_log"], rtol=1e-6, rho_tol=1e-5)
+    res.save(tmp_path / "evidence")              # the summary with its controller fields is JSON
 
 
 @pytest.mark.parametrize("restart", [32, 5])

@@ -299,7 +299,10 @@ class SIMPSolver:
         }
         summary.update(best.summary())
         if callback is not None:
-            summary.update({"n_tail_iters": tail_run, "tail_config": tail, "params_log": params_log})
+            # plain Python values, so that the summary is JSON (a controller
+            # may return numpy scalars)
+            tail_cfg = {k: (v.item() if isinstance(v, np.generic) else v) for k, v in tail.items()}
+            summary.update({"n_tail_iters": tail_run, "tail_config": tail_cfg, "params_log": params_log})
         return OptimizationResult(problem, rho_final, history, summary, best_rho=best.rho)
 
 
