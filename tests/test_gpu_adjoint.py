@@ -48,9 +48,9 @@ def _c3():
 def c3(gpu):
     free, F, ndof, rho, elems = _c3()
     # the adjoint gradient errs to first order in the state's solve error
-    # (its FD check does not): at tol 1e-12 the largest resolvable relative
-    # error is ~2e-7, at 1e-11 it reaches ~1e-6 depending on the
-    # preconditioner's rounding
+    # (its FD check, on the variational form, does not); tol 1e-12 sits
+    # close to the FP64 true residual's rounding floor (~4e-14), below which
+    # the state error stops shrinking
     comp = HipCompliance(DIMS, free, F, rmin=1.5, precision="bf16", tol=1e-12)
     r = torch.tensor(rho, dtype=torch.float64, device=DEV, requires_grad=True)
     C = comp(r)
@@ -65,21 +65,37 @@ def test_adjoint_matches_central_fd(c3):
     base = torch.tensor(rho, dtype=torch.float64, device=DEV)
     c0 = comp.variational(base)
     assert c0 == pytest.approx(c3["C"], rel=1e-9)
-    errs, resolvable = [], []
-    for k in c3["elems"]:
+
+    def fd(k, step):
         rp, rm = base.clone(), base.clone()
-        rp[k] += h
-        rm[k] -= h
-        fd = comp.central_difference(comp.variational_parts(rp), comp.variational_parts(rm), h)
-        errs.append(abs(fd - g[k]) / abs(g[k]))
-        # the difference quotient resolves g_k when the compliance change
-        # 2h|g_k| is well above the ~1e-16 relative noise of FP64 state solves
-        resolvable.append(2 * h * abs(g[k]) >= 1e-10 * c0)
-    errs, resolvable = np.array(errs), np.array(resolvable)
-    assert resolvable.sum() >= 56
+        rp[k] += step
+        rm[k] -= step
+        return comp.central_difference(comp.variational_parts(rp), comp.variational_parts(rm), step)
+
+    errs, rich, resolvable = [], [], []
+    for k in c3["elems"]:
+        f1, f2 = fd(k, h), fd(k, 2 * h)
+        # Richardson: (4 FD(h) - FD(2h)) / 3 removes the O(h^2) truncation
+        fr = (4.0 * f1 - f2) / 3.0
+        errs.append(abs(f1 - g[k]) / abs(g[k]))
+        rich.append(abs(fr - g[k]) / abs(g[k]))
+        # the adjoint gradient errs to first order in the state's error,
+        # which the FP64 true residual bounds only down to its rounding floor
+        # (about 4e-14 relative here: at tol 1e-13 the solve stops at 5.5e-14
+        # and the errors below do not shrink). The measured result is an
+        # absolute error of about 5e-13 C in every g_k, so g_k is resolved to
+        # 1e-6 relative when |g_k| >= 1e-6 C
+        resolvable.append(abs(g[k]) >= 1e-6 * c0)
+        print(f"adjoint FD element {int(k):6d}: rho {rho[k]:.4f} |g|/C {abs(g[k]) / c0:.3e} "
+              f"err(h) {errs[-1]:.2e} err(Richardson h, 2h) {rich[-1]:.2e}")
+    errs, rich, resolvable = np.array(errs), np.array(rich), np.array(resolvable)
+    print(f"adjoint FD: max err(h) {errs.max():.2e}, max err(Richardson) {rich.max():.2e}, "
+          f"resolvable {int(resolvable.sum())} of {errs.size}")
+    assert resolvable.sum() >= 58
     assert errs[resolvable].max() < 1e-6, np.sort(errs[resolvable])[-4:]
     assert np.median(errs) < 2e-7
-    assert errs.max() < 1e-5, np.sort(errs)[-4:]
+    # the rest (|g_k| / C between 1.5e-7 and 1e-6) sit at the floor
+    assert np.all(errs * np.abs(g[c3["elems"]]) / c0 < 2e-12), np.sort(errs * np.abs(g[c3["elems"]]) / c0)[-4:]
 
 
 def test_adjoint_matches_oracle_chain_all_elements(c3):

@@ -132,43 +132,84 @@ def test_graph_replay_equals_direct_launches(gpu, nrhs):
         assert np.array_equal(u0, u1)
 
 
-def _host_cycle(levels, lv, b, degrees, omega, cycle):
-    """The multigrid cycle of GalerkinMatFreeGMG with the damped-Jacobi
-    smoother (multigrid_v4.py:918-932, 963-967, 986-1042) in FP64 on the
-    oracle's Galerkin hierarchy [(free, K, P)]: pre-smoothing from zero,
-    one (V) or two (W) coarse corrections, post-smoothing, exact coarsest
-    solve."""
+def _host_chebyshev(K, rhs, x, Dinv, lam_max, lower_frac, degree):
+    """_chebyshev_steps (multigrid_v4.py:860-889) as the reference writes it:
+    Saad's Algorithm 12.1 with M^-1 = D^-1, updating the iterate x with a
+    full residual rhs - K x at every step."""
+    lam_min = lower_frac * lam_max
+    sigma, delta = 0.5 * (lam_max + lam_min), 0.5 * (lam_max - lam_min)
+    r = rhs - K @ x
+    alpha = 2.0 / (lam_max + lam_min)
+    d = Dinv * r
+    x = x + alpha * d
+    for _ in range(1, degree):
+        r = rhs - K @ x
+        z = Dinv * r
+        a_new = 1.0 / (sigma - delta ** 2 * alpha * 0.25)
+        d = a_new * (z + delta ** 2 * alpha * 0.25 * d)
+        x = x + d
+        alpha = a_new
+    return x
+
+
+def _host_cycle(levels, lv, b, degrees, smooth, cycle):
+    """The multigrid cycle of GalerkinMatFreeGMG (_vcycle / _wcycle,
+    multigrid_v4.py:986-1042) in FP64 on the oracle's Galerkin hierarchy
+    [(free, K, P)]: pre-smoothing from zero, one (V) or two (W) coarse
+    corrections, post-smoothing from the corrected iterate, exact coarsest
+    solve. smooth(lv, K, rhs, x, Dinv, degree) is the smoother."""
     _, K, _ = levels[lv]
     if lv == len(levels) - 1:
         return np.linalg.solve(K.toarray(), b)
     Dinv = 1.0 / K.diagonal()
-    x = np.zeros_like(b)
-    for _ in range(degrees[lv]):
-        x = x + omega * Dinv * (b - K @ x)
+    x = smooth(lv, K, b, np.zeros_like(b), Dinv, degrees[lv])
     P = levels[lv + 1][2]
     for _ in range(2 if cycle == "w" else 1):
-        x = x + P @ _host_cycle(levels, lv + 1, P.T @ (b - K @ x), degrees, omega, cycle)
-    for _ in range(degrees[lv]):
-        x = x + omega * Dinv * (b - K @ x)
-    return x
+        x = x + P @ _host_cycle(levels, lv + 1, P.T @ (b - K @ x), degrees, smooth, cycle)
+    return smooth(lv, K, b, x, Dinv, degrees[lv])
 
 
-@pytest.mark.parametrize("cycle", ["v", "w"])
-def test_jacobi_smoother_and_cycles_match_host_restatement(gpu, cycle):
-    """smoother="jacobi" (omega = min(0.5, 0.9 * 2 / lambda_max of level 0))
-    with a V- or W-cycle: one preconditioner application on the device
-    against the same cycle in FP64 on the oracle's Galerkin operators (FP32
-    products on the device: 2e-5 relative)."""
+def _jacobi(omega):
+    """Damped Jacobi (multigrid_v4.py:918-932, 963-967)."""
+    def smooth(lv, K, b, x, Dinv, degree):
+        for _ in range(degree):
+            x = x + omega * Dinv * (b - K @ x)
+        return x
+    return smooth
+
+
+@pytest.mark.parametrize("smoother,cycle,degrees", [("jacobi", "v", (2, 3)), ("jacobi", "w", (2, 3)),
+                                                    ("chebyshev", "v", (2, 3)), ("chebyshev", "v", (3, 2)),
+                                                    ("chebyshev", "w", (2, 3))])
+def test_smoother_and_cycles_match_host_restatement(gpu, smoother, cycle, degrees):
+    """One preconditioner application on the device against the same cycle
+    in FP64 on the oracle's Galerkin operators (FP32 products on the device:
+    2e-5 relative):
+
+    * smoother="jacobi": omega = min(0.5, 0.9 * 2 / lambda_max of level 0);
+    * smoother="chebyshev": the reference's recurrence on
+      [lambda_max / 30, lambda_max] with each level's lambda_max the device
+      power-iteration estimate times the safety factor 1.1 -- the device
+      runs it in residual-update form, the host as the reference writes it,
+      so a wrong coefficient, bound or scaling on either side shows here."""
     dims = (16, 8, 8)
     free, F, ndof = cantilever_bc(dims, point_support=True)
     rng = np.random.default_rng(12)
     E = 1e-3 + rng.uniform(0.0, 1.0, int(np.prod(dims))) ** 3
-    s = HipStateSolver(dims, free, F, precision="fp32", n_levels=3, fine_degree=2, coarse_degree=3,
-                       smoother="jacobi", cycle=cycle)
+    fine_deg, coarse_deg = degrees
+    s = HipStateSolver(dims, free, F, precision="fp32", n_levels=3, fine_degree=fine_deg, coarse_degree=coarse_deg,
+                       smoother=smoother, cycle=cycle)
     E64 = torch.from_numpy(E).to(DEV)
     s.set_modulus(E64, E64.float(), float(E.max()))
     g = s.gmg
-    omega = min(0.5, 0.9 * 2.0 / g.levels[0].lam)
+    if smoother == "jacobi":
+        smooth = _jacobi(min(0.5, 0.9 * 2.0 / g.levels[0].lam))
+    else:
+        assert g.safety == 1.1 and g.frac == pytest.approx(1.0 / 30.0)
+        lams = [g.levels[lv].lam * g.safety for lv in range(2)]
+
+        def smooth(lv, K, b, x, Dinv, degree):
+            return _host_chebyshev(K, b, x, Dinv, lams[lv], 1.0 / 30.0, degree)
     K = oracle.assemble_k(oracle.edof_table_3d(*dims), E, ndof)[free, :][:, free]
     from oracle.gmg import galerkin_coarse_operators, level_dims
     levels = galerkin_coarse_operators(K, level_dims(*dims, 3), free)
@@ -177,7 +218,7 @@ def test_jacobi_smoother_and_cycles_match_host_restatement(gpu, cycle):
     r[free] = b
     st = torch.cuda.current_stream().cuda_stream
     z = h(g.apply(torch.from_numpy(r.astype(np.float32)).to(DEV), st))
-    want = _host_cycle(levels, 0, b, [2, 3, 3], omega, cycle)
+    want = _host_cycle(levels, 0, b, [fine_deg, coarse_deg, coarse_deg], smooth, cycle)
     assert rel_l2(z[free], want) < 2e-5
     # and as a preconditioner: the solve converges to the direct solution
     st_ = s.solve()

@@ -199,14 +199,27 @@ def test_matfree_matches_assembled():
     assert np.allclose(d[free], Kff.diagonal(), rtol=1e-14)
 
 
-@pytest.mark.parametrize("name", ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2"])
+PLAIN_GOLDEN = ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2", "c2r_line_32x16x16", "c0d_pressure_12x6x6",
+                "tp_portal_bridge_24x12x10"]
+
+
+def golden_case(name: str) -> dict:
+    """The spec and run_simp parameters a golden fixture was written with."""
+    import json
+
+    from conftest import GOLDEN
+
+    return json.loads((GOLDEN / f"{name}.json").read_text())["case"]
+
+
+@pytest.mark.parametrize("name", PLAIN_GOLDEN)
 def test_run_simp_matches_reference_golden(golden, name):
     g = golden(name)
-    cfg = {"c0_cantilever_24x12x6": ((24, 12, 6), 0.30, 1.5, 3),
-           "c0b_block_10x5x3_rmin2": ((10, 5, 3), 0.40, 2.0, 6)}[name]
-    (dims, vf, rmin, iters) = cfg
-    r = oracle.run_simp(*dims, vf, g["free"], g["F"], rmin=rmin, move=0.2, max_iter=iters, min_iter=2,
-                        tol=1e-3)
+    case = golden_case(name)
+    sp, p = case["spec"], case["params"]
+    dims = (sp["nelx"], sp["nely"], sp["nelz"])
+    r = oracle.run_simp(*dims, sp["volfrac"], g["free"], g["F"], rmin=p["rmin"], move=p["move"],
+                        max_iter=p["max_iter"], min_iter=p["min_iter"], tol=p["tol"])
     assert np.allclose(r["compliance_history"], g["compliance_history"], rtol=1e-9, atol=0)
     assert r["n_iter"] == int(g["n_iter"])
     assert np.abs(r["rho_final"] - g["rho_final"]).max() < 1e-9
@@ -253,26 +266,72 @@ def test_run_simp_controlled_matches_reference_golden(golden, case):
                          r["params_log"], rtol=1e-9, rho_tol=1e-9)
 
 
-@pytest.mark.parametrize("name", ["c0_cantilever_24x12x6", "c0b_block_10x5x3_rmin2"])
+def _ns_spec(spec: dict):
+    """A reference ProblemSpec dict as the duck-typed object the oracle reads."""
+    from types import SimpleNamespace as NS
+
+    ns = NS(**{k: v for k, v in spec.items() if k not in ("supports", "loads")},
+            supports=[NS(**{k: v for k, v in d.items() if k != "type"}) for d in spec["supports"]],
+            loads=[NS(**{k: v for k, v in d.items() if k != "type"}) for d in spec["loads"]])
+    for ld in ns.loads:
+        if not hasattr(ld, "edge"):
+            for k in ("fx", "fy", "fz"):
+                setattr(ld, k, getattr(ld, k, 0.0))
+    for su in ns.supports:
+        su.constraint = getattr(su, "constraint", "fixed")
+    return ns
+
+
+@pytest.mark.parametrize("name", [n for n in PLAIN_GOLDEN if n != "c2r_line_32x16x16"])
 def test_generate_bc_matches_reference_golden(golden, name):
     """Fixed/free DOFs and the load vector equal those the reference's own
     generate_bc produced for the golden cases (bc_generator.py:381-447)."""
-    import json
-    from types import SimpleNamespace as NS
-
-    from conftest import GOLDEN
-
     g = golden(name)
-    spec = json.loads((GOLDEN / f"{name}.json").read_text())["case"]["spec"]
-    ns = NS(**{k: v for k, v in spec.items() if k not in ("supports", "loads")},
-            supports=[NS(**d) for d in spec["supports"]], loads=[NS(**d) for d in spec["loads"]])
-    for ld in ns.loads:
-        for k in ("fx", "fy", "fz"):
-            setattr(ld, k, getattr(ld, k, 0.0))
-    fixed, free, F, ndof = generate_bc(ns)
+    fixed, free, F, ndof = generate_bc(_ns_spec(golden_case(name)["spec"]))
     assert np.array_equal(free, g["free"])
     assert np.array_equal(F, g["F"])
     assert fixed.size + free.size == ndof == F.size
+
+
+def test_generate_bc_cases_match_reference(golden):
+    """Face pressure on all six faces of a non-unit mesh (trapezoidal
+    tributary areas, bc_generator.py:229-311), point and face supports of
+    every constraint kind, several loads on one node, load points halfway
+    between nodes: the oracle and the product's generate_bc both give the
+    reference's fixed DOFs and load vector."""
+    import json
+
+    from conftest import GOLDEN
+    from topogpu_amd.bc import generate_bc as product_generate_bc
+    from topogpu_amd.problem_spec import ProblemSpec
+
+    g = golden("bc_cases")
+    specs = json.loads((GOLDEN / "bc_cases.json").read_text())["specs"]
+    assert len(specs) == 14
+    for i, spec in enumerate(specs):
+        fixed, free, F, ndof = generate_bc(_ns_spec(spec))
+        assert np.array_equal(fixed, g[f"fixed_{i}"]), i
+        assert np.array_equal(F, g[f"F_{i}"]), i
+        b = product_generate_bc(ProblemSpec.from_dict(spec))
+        assert np.array_equal(b.fixed_dofs, g[f"fixed_{i}"]), i
+        assert np.allclose(b.F, g[f"F_{i}"], rtol=1e-15, atol=1e-16), i
+
+
+def test_oc_mean_matches_reference_fixture(golden):
+    """The GPU loop's mean-volume OC (src/topogpu/solver.py:94-117) on
+    inputs that reach each exit of the reference's own function (early stop,
+    bracket doubled to its 1e40 cap, all 80 bisections above or below the
+    target, clamped dc >= 0, densities at the bounds, a rho_min above 1e-3):
+    the oracle's result equals the reference's bit for bit."""
+    import json
+
+    from conftest import GOLDEN
+
+    g = golden("oc_mean_cases")
+    cases = json.loads((GOLDEN / "oc_mean_cases.json").read_text())["cases"]
+    for i, c in enumerate(cases):
+        got = oracle.oc_update_mean(g[f"rho_{i}"], g[f"dc_{i}"], c["volfrac"], c["move"], c["rho_min"])
+        assert np.array_equal(got, g[f"rho_new_{i}"]), c["name"]
 
 
 @pytest.mark.parametrize("nf,nc", [(8, 4), (7, 3), (3, 1), (1, 1), (2, 1)])
@@ -303,3 +362,37 @@ def test_galerkin_hierarchy_small():
     # the x=0 face stays fixed on every level
     fr1 = levels[1][0]
     assert 0 not in fr1 and 1 not in fr1 and 2 not in fr1
+
+
+def test_coarse_free_injection_matches_reference_fixture(golden):
+    """Coarse free DOFs by injection against the reference's own
+    _coarse_free_dofs_injection (solver_v2.py:717-764) on seeded free sets,
+    including axes one element thick (the flat index past the end of the
+    line)."""
+    import json
+
+    from conftest import GOLDEN
+
+    g = golden("injection_cases")
+    cases = json.loads((GOLDEN / "injection_cases.json").read_text())["cases"]
+    thin = 0
+    for i, c in enumerate(cases):
+        got = gmg.coarse_free_injection(tuple(c["fine"]), tuple(c["coarse"]), g[f"free_{i}"])
+        assert np.array_equal(got, g[f"coarse_free_{i}"]), c
+        thin += any(n == 1 for n in c["fine"])
+    assert thin >= 4
+
+
+def test_scalar_prolongation_matches_reference_fixture(golden):
+    """The row-normalised trilinear prolongation against the reference's
+    _build_scalar_prolongation (solver_v2.py:62-149) on even, odd and
+    one-element axes."""
+    import json
+
+    from conftest import GOLDEN
+
+    g = golden("prolongation_cases")
+    cases = json.loads((GOLDEN / "prolongation_cases.json").read_text())["cases"]
+    for i, c in enumerate(cases):
+        P = gmg.scalar_prolongation(tuple(c["fine"]), tuple(c["coarse"])).toarray()
+        assert np.allclose(P, g[f"P_{i}"], rtol=0, atol=1e-15), c

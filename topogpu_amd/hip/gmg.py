@@ -284,9 +284,14 @@ class GMG:
 
     # --------------------------------------------------------------- cycle
     def _cheb_coeffs(self, lv: int, degree: int):
-        """Per-step (a1, a2) of the recurrence alpha_0 = 1/sigma,
-        alpha_k = 1/(sigma - delta^2 alpha_{k-1}/4), d_k = alpha_k z + alpha_k
-        delta^2 alpha_{k-1}/4 d_{k-1} (multigrid_v4.py:870-887).
+        """Per-step (a1, a2) of the steps s_k = a1 D^-1 r + a2 s_{k-1} of the
+        recurrence of multigrid_v4.py:870-887: alpha_0 = 1/sigma,
+        alpha_k = 1/(sigma - delta^2 alpha_{k-1}/4), and the reference's
+        vector d, which is z_0 = D^-1 r_0 after the first step (x moves by
+        alpha_0 z_0) and the step itself afterwards, d_k = alpha_k (z_k +
+        delta^2 alpha_{k-1}/4 d_{k-1}). So in terms of steps the momentum
+        coefficient is alpha_1 delta^2 / 4 at k = 1 (s_0 = alpha_0 z_0) and
+        alpha_k delta^2 alpha_{k-1} / 4 for k >= 2.
 
         Damped Jacobi is the same residual-update sequence with every step
         d_k = omega D^-1 r_{k-1}: (omega, 0) for each of the degree steps."""
@@ -300,9 +305,9 @@ class GMG:
         sigma, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
         alpha = 1.0 / sigma
         out = [(alpha, 0.0)]
-        for _ in range(1, degree):
+        for k in range(1, degree):
             a_new = 1.0 / (sigma - delta * delta * alpha * 0.25)
-            out.append((a_new, a_new * delta * delta * alpha * 0.25))
+            out.append((a_new, a_new * delta * delta * 0.25 * (1.0 if k == 1 else alpha)))
             alpha = a_new
         return out
 
