@@ -178,3 +178,30 @@ def test_passive_mask_matches_bruteforce():
     assert np.array_equal(got, want) and (got == 1).sum() == 10 and (got == 2).sum() == 6
     spec.passive_regions = [CircularRegion(100.0, 100.0, 1.0, "void")]
     assert bc.generate_bc(spec).passive_mask is None
+
+
+def test_solver_v4_contract_validation():
+    """HipSolverV4 rejects what SolverV4 rejects (solver_v4.py:66-82), with
+    the same messages, before touching the GPU."""
+    from topogpu_amd.hip.solver_v4 import HipSolverV4, structured_edof
+
+    import oracle
+
+    dims = (4, 3, 2)
+    assert np.array_equal(structured_edof(dims), oracle.edof_table_3d(*dims))
+    ndof = 3 * 5 * 4 * 3
+    base = dict(free=np.arange(ndof), F=np.ones(ndof), ndof=ndof, enable_matrix_free=True, grid_dims=dims)
+    with pytest.raises(ValueError, match="requires grid_dims"):
+        HipSolverV4(**{**base, "grid_dims": None})
+    with pytest.raises(ValueError, match="requires enable_matrix_free=True"):
+        HipSolverV4(**{**base, "enable_matrix_free": False})
+    with pytest.raises(ValueError, match="gmg_fine_smoother must be one of"):
+        HipSolverV4(**base, gmg_fine_smoother="fp16")
+    with pytest.raises(ValueError, match="gmg_outer_solver must be"):
+        HipSolverV4(**base, gmg_outer_solver="gmres")
+    with pytest.raises(ValueError, match="edof must be the structured"):
+        HipSolverV4(**base, edof=oracle.edof_table_3d(*dims)[::-1])
+    with pytest.raises(ValueError, match="KE_UNIT"):
+        HipSolverV4(**base, KE_UNIT=2.0 * oracle.KE_UNIT_3D)
+    with pytest.raises(ValueError, match="ndof"):
+        HipSolverV4(**{**base, "ndof": ndof + 3})

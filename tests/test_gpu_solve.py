@@ -420,7 +420,10 @@ def test_passive_regions_enforced_when_asked(gpu):
     assert res.summary["n_iter"] == 3
     off = tg.SIMPSolver(**kw).solve(prob)
     eng = HipSIMP(prob.nel, b.free_dofs, b.F, volfrac=0.3, rmin=1.5, move=0.2, tol=1e-8, passive=b.passive_mask)
-    eng.step()
+    rho0 = eng.rho.cpu().numpy()
+    assert np.all(rho0[mask == 1] == 1e-3) and np.all(rho0[mask == 2] == 1.0)   # the start is clamped too
+    step = eng.step()
     rho = eng.rho.cpu().numpy()
     assert np.all(rho[mask == 1] == 1e-3) and np.all(rho[mask == 2] == 1.0)
+    assert step.change == np.abs(rho - rho0).max()          # the change of the clamped design
     assert not np.array_equal(res.rho_final, off.rho_final)

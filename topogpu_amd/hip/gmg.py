@@ -52,6 +52,9 @@ KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
 _EP_VEC_TRAFFIC = {L.TG_EP_STORE: 4, L.TG_EP_DOT: 4, L.TG_EP_RESID: 8, L.TG_EP_RESID_NORM: 8, L.TG_EP_CHEB: 20,
                    L.TG_EP_CHEB_R: 24, L.TG_EP_RESID_UPDATE: 8, L.TG_EP_CHEB_FIRST: 16, L.TG_EP_RESID_FIRST: 12,
                    L.TG_EP_CHEB_LAST: 16}
+_COEF_BYTES = {L.TG_OP_STENCIL_F32: 972, L.TG_OP_STENCIL_BF16: 486}
+EP_NAMES = {0: "STORE", 1: "DOT", 2: "RESID", 3: "RESID_NORM", 4: "CHEB", 5: "CHEB_R", 6: "RESID_UPDATE",
+            7: "CHEB_FIRST", 8: "RESID_FIRST", 9: "CHEB_LAST", -1: "-"}
 
 
 def hex_bytes(n_dof: int, n_elem: int, mode: int) -> float:
@@ -61,14 +64,39 @@ def hex_bytes(n_dof: int, n_elem: int, mode: int) -> float:
     return 4.0 * n_dof + 4.0 * n_elem + n_dof / 3.0 + _EP_VEC_TRAFFIC[mode] * n_dof
 
 
+def op_bytes(kind: int, n_dof: int, n_elem: int, mode: int) -> float:
+    """Algorithmic bytes of one operator launch of any level: the level-0
+    count, with E read on the fine grid (8 fine elements per coarse one) for
+    the implicit level-1 operator, and the 27-point coefficients (243 per
+    node) instead of E for a stencil level."""
+    if kind in _COEF_BYTES:
+        return hex_bytes(n_dof, 0, mode) + _COEF_BYTES[kind] * (n_dof / 3.0)
+    if kind in (L.TG_OP_Q8_F32, L.TG_OP_Q8_BF16, L.TG_OP_Q8_BF16X3):
+        return hex_bytes(n_dof, 8 * n_elem, mode)
+    return hex_bytes(n_dof, n_elem, mode)
+
+
+def kernel_key(lv: int, kind: int) -> str:
+    return f"L{lv} {KERNEL_NAMES[kind]}"
+
+
 def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: int, nrhs: int = 1) -> None:
-    """tg_apply32 over nrhs right-hand sides, accounted when a timer is
-    attached (level 0 only)."""
-    if timer is None or lv != 0:
+    """tg_apply32 over nrhs right-hand sides, accounted (level, operator,
+    epilogue) when a timer is attached."""
+    if timer is None:
         L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream)
         return
-    timer.launch(KERNEL_NAMES[op.kind], nrhs * hex_bytes(n_dof, n_elem, ep.mode),
+    timer.launch(kernel_key(lv, op.kind), nrhs * op_bytes(op.kind, n_dof, n_elem, ep.mode),
                  lambda: L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream), ep.mode)
+
+
+def launch_transfer(timer, name: str, lv: int, nbytes: float, *args) -> None:
+    """tg_restrict32_b / tg_prolong32_b between levels lv and lv + 1,
+    accounted when a timer is attached."""
+    if timer is None:
+        L.call(name, *args)
+        return
+    timer.launch(f"L{lv} {name[3:-2]}", nbytes, lambda: L.call(name, *args))
 
 
 class GMG:
@@ -194,7 +222,7 @@ class GMG:
         self.scal = torch.zeros(8, dtype=torch.float64, device=device)
         self._emax_at_estimate = float("nan")
         self.is_setup = False
-        self.timer = None          # optional KernelTimer (level-0 launches)
+        self.timer = None          # optional KernelTimer (operator and transfer launches)
         self.E64 = self.E32 = None
 
     # ------------------------------------------------------------------ ops
@@ -357,7 +385,7 @@ class GMG:
         of its residual to the next level's right-hand side."""
         Lv, C = self.levels[lv], self.levels[lv + 1]
         self._presmooth(lv, rhs, self._degree(lv), stream)
-        L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
+        self._restrict(lv, stream)
 
     def _vcycle_up(self, lv: int, stream):
         """Second half: the coarser cycle, prolongation, post-smoothing. A
@@ -367,14 +395,27 @@ class GMG:
         Lv, C = self.levels[lv], self.levels[lv + 1]
         ec = self._vcycle(lv + 1, C.rhs, stream)
         if self.cycle == "w":
-            L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
+            self._prolong(lv, ec, stream)
             self._launch(lv, self.op(lv, accurate=True), Lv.c, None,
                          L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=Lv.da, x=Lv.x, r=Lv.r, a1=0.0, a2=0.0, a3=1.0), stream)
-            L.call("tg_restrict32_b", Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
+            self._restrict(lv, stream)
             ec = self._vcycle(lv + 1, C.rhs, stream)
-        L.call("tg_prolong32_b", Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
+        self._prolong(lv, ec, stream)
         self._postsmooth(lv, self._degree(lv), stream)
         return Lv.x
+
+    def _restrict(self, lv: int, stream) -> None:
+        """C.rhs = M_c P^T r of level lv (reads r, the coarse mask; writes the
+        coarse right-hand side)."""
+        Lv, C = self.levels[lv], self.levels[lv + 1]
+        launch_transfer(self.timer, "tg_restrict32_b", lv, self.nrhs * (4.0 * Lv.n + 4.0 * C.n + C.nodes),
+                        Lv.grid, C.grid, L.ptr(C.mask), self.nrhs, L.ptr(Lv.r), L.ptr(C.rhs), stream)
+
+    def _prolong(self, lv: int, ec, stream) -> None:
+        """Lv.c = M P ec (reads ec, the fine mask; writes the correction)."""
+        Lv, C = self.levels[lv], self.levels[lv + 1]
+        launch_transfer(self.timer, "tg_prolong32_b", lv, self.nrhs * (4.0 * C.n + 4.0 * Lv.n + Lv.nodes),
+                        Lv.grid, C.grid, L.ptr(Lv.mask), self.nrhs, L.ptr(ec), L.ptr(Lv.c), 0, stream)
 
     def _vcycle(self, lv: int, rhs, stream):
         Lv = self.levels[lv]

@@ -73,7 +73,7 @@ def capture(key, issue, owner, device) -> LaunchGraph:
     """Capture the launches issue(stream) makes on a fresh side stream.
 
     owner.timer (a KernelTimer or None) is swapped for a recorder while
-    capturing, so the graph knows the level-0 launches of one replay."""
+    capturing, so the graph knows the accounted launches of one replay."""
     side = torch.cuda.Stream(device=device)
     st = side.cuda_stream
     rec = _Recorder()
@@ -96,7 +96,7 @@ def capture(key, issue, owner, device) -> LaunchGraph:
 
 
 def account(timer, graph: LaunchGraph) -> None:
-    """Add one replay's level-0 launches to a KernelTimer's counters."""
+    """Add one replay's accounted launches to a KernelTimer's counters."""
     if timer is None:
         return
     for k, n in graph.counts.items():

@@ -71,7 +71,12 @@ class HipSIMP:
         self.set_rmin(self.rmin)
         f64 = dict(dtype=torch.float64, device=self.dev)
         rho = np.full(self.n_elem, self.volfrac) if rho0 is None else np.asarray(rho0, dtype=np.float64)
-        self.rho = torch.from_numpy(rho.copy()).to(self.dev)
+        rho = rho.copy()
+        if pm is not None:
+            # the passive elements start at their bounds, as after every update
+            rho[pm == 1] = self._passive_lo()
+            rho[pm == 2] = 1.0
+        self.rho = torch.from_numpy(rho).to(self.dev)
         self.rho_new = torch.zeros(self.n_elem, **f64)
         self.rho_f = torch.zeros(self.n_elem, **f64)
         self.rho_phys = torch.zeros(self.n_elem, **f64)
@@ -177,8 +182,7 @@ class HipSIMP:
                 break
         L.call("tg_oc_mean_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.move, self.rho_min,
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
-        self._apply_passive(self.rho_new)
-        change = float(self.scal[0].item())
+        change = self._apply_passive(self.rho_new)
         n_steps = int(self.oc_state[3].item())
         self.rho, self.rho_new = self.rho_new, self.rho
         return change, n_steps
@@ -207,19 +211,24 @@ class HipSIMP:
                 break
         L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
-        self._apply_passive(self.rho_new)
-        change = float(self.scal[0].item())
+        change = self._apply_passive(self.rho_new)
         n_steps = int(self.oc_state[4].item())
         self.rho, self.rho_new = self.rho_new, self.rho
         return change, n_steps
 
-    def _apply_passive(self, rho) -> None:
+    def _passive_lo(self) -> float:
+        return self.rho_min if self.formulation == "solverv4" else 1.0e-3
+
+    def _apply_passive(self, rho_new) -> float:
         """Void elements to the lower density bound, solid ones to 1
         (apply_passive_mask, bc_generator.py:454-470; rho_min = 1e-3 there,
-        the filtered OC's bound)."""
-        if self.passive is not None:
-            lo = self.rho_min if self.formulation == "solverv4" else 1.0e-3
-            L.call("tg_passive64", self.n_elem, L.ptr(self.passive), L.ptr(rho), lo, self.stream)
+        the filtered OC's bound). Returns max |rho_new - rho| of the design
+        the next iteration uses: the finalize kernel's value, or, with
+        passive elements, recomputed after the clamp."""
+        if self.passive is None:
+            return float(self.scal[0].item())
+        L.call("tg_passive64", self.n_elem, L.ptr(self.passive), L.ptr(rho_new), self._passive_lo(), self.stream)
+        return float((rho_new - self.rho).abs().max().item())
 
     def step(self) -> StepResult:
         self.physical()

@@ -5,7 +5,9 @@ on the stream the kernel is launched on (torch's current stream, which every
 libtopohip call here receives), so the measured durations are those of the
 kernels themselves. Events cost host time per launch, so a timed benchmark
 region uses ``events=False``: launches and algorithmic bytes are counted per
-(kernel, epilogue mode) and nothing else is done.
+(kernel, epilogue mode) and nothing else is done. The last launch of each
+key is kept as a closure (``replay``), so that its duration can be measured
+afterwards by launching it again back to back.
 """
 
 from __future__ import annotations
@@ -21,10 +23,12 @@ class KernelTimer:
         self.records = defaultdict(list)          # name -> [(start, end, bytes)]
         self.counts = defaultdict(int)            # (name, mode) -> launches
         self.bytes = defaultdict(float)           # (name, mode) -> bytes
+        self.replay = {}                          # (name, mode) -> (fn, bytes of one launch)
 
     def launch(self, name: str, nbytes: float, fn, mode: int = -1):
         self.counts[(name, mode)] += 1
         self.bytes[(name, mode)] += float(nbytes)
+        self.replay[(name, mode)] = (fn, float(nbytes))
         if not self.events:
             fn()
             return
@@ -56,3 +60,4 @@ class KernelTimer:
         self.records.clear()
         self.counts.clear()
         self.bytes.clear()
+        self.replay.clear()

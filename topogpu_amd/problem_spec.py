@@ -2,8 +2,10 @@
 
 Same field names, defaults, validation messages and JSON round trip as the
 reference's ``ProblemSpec`` (problem_spec.py:31-313) for supports, loads and
-the structured mesh. Passive regions are carried in the schema but not
-honoured by the ``hip`` backend, which raises if any are present.
+the structured mesh. Passive regions are carried in the schema;
+``generate_bc`` derives their per-element mask, which the ``hip`` loop
+enforces after each OC update when asked (``apply_passive=True``) and, like
+the reference loops, leaves unapplied otherwise.
 """
 
 from __future__ import annotations

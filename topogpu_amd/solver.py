@@ -257,7 +257,13 @@ class SIMPSolver:
                 state = C.step_state(iteration, comp_hist, window, conv, rho_phys, gray, (nx, ny, nz), best, ctl)
                 action = callback(state, rho_new)
                 if action:
+                    restart = bool(action.get("restart", False)) and best.is_valid
                     rho_new = C.apply_action(action, ctl, rho_new, best.rho, best.is_valid)
+                    if restart:
+                        # the displacements belong to another design: the
+                        # next solve starts from zero (pub_simp_solver.py:
+                        # 588-589 invalidates the solver here)
+                        engine.fea.reset_warm_start()
                 # the next design is the controller's (it may also have
                 # changed the array in place)
                 to_device(rho_new)
@@ -282,6 +288,7 @@ class SIMPSolver:
             ctl.penal, ctl.rmin = float(tail.get("penal", ctl.penal)), float(tail.get("rmin", ctl.rmin))
             ctl.move, ctl.beta = float(tail.get("move", ctl.move)), float(tail.get("beta", ctl.beta))
             engine.set_rmin(ctl.rmin)
+            engine.fea.reset_warm_start()             # pub_simp_solver.py:622
             for k in range(1, int(tail["tail_iters"]) + 1):
                 run_step(iteration + k, "tail")
                 tail_run = k
