@@ -163,6 +163,20 @@ int tg_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, void* stream);
 int tg_apply64(const tg_grid* g, const uint8_t* mask, const double* E, const double* u,
                const double* rhs, double* y, float* y32, double* sumsq, void* stream);
 
+/* tg_apply64 without the library's allocated workspace: the reduction
+ * partials of sumsq live in a static device array of the library (one per
+ * GPU, 65,536 FP64 partials, held until the process ends), so the call
+ * itself allocates nothing and needs no workspace argument. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc); tg_init, once at set-up, resolves the array's address. Grids
+ * of more than 65,536 node tiles of 4 x 4 x 16 nodes (about 16M nodes) are
+ * rejected with TG_ERR_ARG. Calls that overlap on one GPU share the array
+ * and race; calls ordered on one stream do not. Results are bit-identical
+ * to tg_apply64's. */
+int tg_apply64_noalloc(const tg_grid* g, const uint8_t* mask, const double* E, const double* u,
+                       const double* rhs, double* y, float* y32, double* sumsq, void* stream);
+
 /* Inverse diagonal of M K(E) M (fixed DOFs -> 1). Replaces
  * MatrixFreeKff.extract_diagonal (solver_v2.py:1229-1252) and the D^-1 setup
  * (multigrid_v4.py:684-686). dinv64 or dinv32 may be NULL. */

@@ -392,6 +392,7 @@ extern "C" int tg_init(const double* ke) {
     g_ws[dev].flags = (int*)((char*)p + sizeof(double) * kMaxPartials);
     TG_HIP_CHECK(hipMemset(g_ws[dev].flags, 0, 256));
   }
+  if (int rc = init_static_scratch(dev)) return rc;
   TG_HIP_CHECK(hipDeviceSynchronize());
   g_ready[dev] = true;
   return TG_OK;
@@ -530,12 +531,16 @@ extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float
 }
 
 namespace tg {
-int finalize_batched(int K, int seg, int nseg, double* out0, double* out1, hipStream_t s) {
-  TG_REQUIRE(K >= 1 && K <= 2 && seg >= 1 && nseg >= 1 && (size_t)K * seg * nseg <= (size_t)kMaxPartials,
+int finalize_batched_at(const double* partials, size_t capacity, int K, int seg, int nseg, double* out0,
+                        double* out1, hipStream_t s) {
+  TG_REQUIRE(partials && K >= 1 && K <= 2 && seg >= 1 && nseg >= 1 && (size_t)K * seg * nseg <= capacity,
              "bad reduction shape");
-  hipLaunchKernelGGL(k_finalize, dim3(nseg), dim3(256), 0, s, K, seg, workspace().partials, out0, out1);
+  hipLaunchKernelGGL(k_finalize, dim3(nseg), dim3(256), 0, s, K, seg, partials, out0, out1);
   TG_LAUNCH_CHECK();
   return TG_OK;
+}
+int finalize_batched(int K, int seg, int nseg, double* out0, double* out1, hipStream_t s) {
+  return finalize_batched_at(workspace().partials, (size_t)kMaxPartials, K, seg, nseg, out0, out1, s);
 }
 int finalize_sums(int K, int nb, double* out0, double* out1, hipStream_t s) {
   return finalize_batched(K, nb, 1, out0, out1, s);

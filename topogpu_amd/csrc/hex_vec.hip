@@ -326,7 +326,29 @@ int launch_node(const Grid& g, const uint8_t* mask, W w, const T* u, const Ep& e
   return ep.reduces() ? finalize_batched(1, nb, nrhs, dot, nullptr, s) : TG_OK;
 }
 
+// Scratch of the allocation-free FP64 residual: the per-block partials of
+// its reduction, in a file-scope device array (one copy per GPU, held until
+// the process ends) instead of the workspace tg_init allocates. Host code
+// never uses the array's name as an address: tg_init resolves each device's
+// copy with hipGetSymbolAddress into g_scratch[dev], and the kernels receive
+// that pointer.
+constexpr int kStaticPartials = 1 << 16;   // 512 KiB: grids up to 65,536 node tiles
+static __device__ __align__(256) double scratch[kStaticPartials];
+constexpr int kScratchDevices = 64;
+double* g_scratch[kScratchDevices] = {};
+
 }  // namespace
+
+int init_static_scratch(int dev) {
+  TG_REQUIRE(dev >= 0 && dev < kScratchDevices, "device %d out of range", dev);
+  if (g_scratch[dev]) return TG_OK;
+  void* ptr = nullptr;
+  TG_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  TG_REQUIRE(ptr != nullptr, "static scratch has no device address");
+  g_scratch[dev] = static_cast<double*>(ptr);
+  return TG_OK;
+}
+
 }  // namespace tg
 
 using namespace tg;
@@ -439,6 +461,29 @@ extern "C" int tg_apply64_b(const tg_grid* gp, const uint8_t* mask, const double
 extern "C" int tg_apply64(const tg_grid* gp, const uint8_t* mask, const double* E, const double* u,
                           const double* rhs, double* y, float* y32, double* sumsq, void* stream) {
   return tg_apply64_b(gp, mask, E, 1, u, rhs, y, y32, sumsq, stream);
+}
+
+extern "C" int tg_apply64_noalloc(const tg_grid* gp, const uint8_t* mask, const double* E, const double* u,
+                                  const double* rhs, double* y, float* y32, double* sumsq, void* stream) {
+  TG_NEED_INIT();
+  if (int rc = check_grid(gp)) return rc;
+  TG_REQUIRE(mask && E && u && (y || y32), "tg_apply64_noalloc: NULL argument");
+  int dev = -1;
+  TG_HIP_CHECK(hipGetDevice(&dev));
+  TG_REQUIRE(dev >= 0 && dev < kScratchDevices && g_scratch[dev], "tg_apply64_noalloc: no scratch on device %d",
+             dev);
+  const Grid g = to_grid(gp);
+  const int nb = node_tiles(g);
+  TG_REQUIRE(nb <= kStaticPartials,
+             "tg_apply64_noalloc: %d node tiles exceed the static scratch of %d partials (grids up to %d nodes)",
+             nb, kStaticPartials, kStaticPartials * NT);
+  Ep64 e{rhs, y, y32, sumsq != nullptr};
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL((k_node_op<double, HexW<double>, Ep64>), dim3(nb, 1), dim3(NT), 0, s, g, mask, HexW<double>{E},
+                     u, e, sumsq, Workspace{g_scratch[dev], nullptr});
+  TG_LAUNCH_CHECK();
+  return e.reduces() ? finalize_batched_at(g_scratch[dev], (size_t)kStaticPartials, 1, nb, 1, sumsq, nullptr, s)
+                     : TG_OK;
 }
 
 extern "C" int tg_mfma_probe(const uint16_t* a, const uint16_t* b, float* d, void* stream) {

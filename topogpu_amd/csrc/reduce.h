@@ -69,5 +69,11 @@ int finalize_sums(int K, int nblocks, double* out0, double* out1, hipStream_t s)
 // As finalize_sums for a grid of nseg slices of seg blocks each (one slice
 // per right-hand side): out_k[r] sums slice r. One block per slice.
 int finalize_batched(int K, int seg, int nseg, double* out0, double* out1, hipStream_t s);
+// The same over partials at a given device address (capacity doubles).
+int finalize_batched_at(const double* partials, size_t capacity, int K, int seg, int nseg, double* out0,
+                        double* out1, hipStream_t s);
+// Resolve the device address of the static scratch of the allocation-free
+// entry point for device dev (defined in hex_vec.hip; called by tg_init).
+int init_static_scratch(int dev);
 
 }  // namespace tg

@@ -56,6 +56,7 @@ _SIGS = {
     "tg_graph_launch": [_vp, _vp],
     "tg_graph_destroy": [_vp],
     "tg_apply64": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tg_apply64_noalloc": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_mfma_probe": [_vp, _vp, _vp, _vp],
     "tg_diag_inv_hex": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp, _vp],
     "tg_diag_inv_stored": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
