@@ -197,14 +197,17 @@ def saturated_kernel_times(counter, steps: int, reps: int = 50) -> dict:
 
 
 def kernel_probe(precision: str, reps: int = 20) -> dict:
-    """Reference copy of known size, then the three heaviest level-0 kernels
-    on live operands (the smoother's CHEB_R and CHEB_FIRST steps and the PCG
-    product, within a few percent of each other in device time per step):
-    what a counter run scales its byte counts with."""
+    """For hardware-counter runs: two SIMP iterations of the 1M-element
+    cantilever, a 256 MiB device copy of known size (what a counter run
+    scales its byte counts with), then reps launches each, on the live
+    operands, of the level-0 smoother steps CHEB_R, CHEB_FIRST, RESID_UPDATE
+    and CHEB_LAST, the split-BF16 PCG product (DOT) and the plain product
+    (STORE), the level-1 Galerkin operator's two smoothing steps, and the
+    CHEB_R step of the stencil levels 2 and 3, in that order."""
     import torch
 
     from topogpu_amd.hip import lib as L
-    from topogpu_amd.hip.gmg import hex_bytes
+    from topogpu_amd.hip.gmg import EP_NAMES, kernel_key, op_bytes
 
     eng = make_engine("c2", precision)
     for _ in range(2):
@@ -216,26 +219,37 @@ def kernel_probe(precision: str, reps: int = 20) -> dict:
     torch.cuda.synchronize()
     L.call("tg_copy32", src.numel(), L.ptr(dst), L.ptr(src), st)   # k_copy32: reads and writes copy_bytes
     g = eng.fea.gmg
-    lv0 = g.levels[0]
-    u, r, x, d = lv0.da.clone(), lv0.r.clone(), lv0.x.clone(), torch.zeros_like(lv0.da)
-    op = g.op(0)
-    ep = L.ep(L.TG_EP_CHEB_R, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1)
-    for _ in range(reps):
-        L.call("tg_apply32", op, L.ptr(u), None, ep, st)
-    fine = eng.fea
-    y = torch.zeros_like(lv0.da)
-    dot = torch.zeros(1, dtype=torch.float64, device=y.device)
-    for _ in range(reps):
-        L.call("tg_apply32", fine.fine_op(), L.ptr(u), L.ptr(y), L.ep(L.TG_EP_DOT, dot=dot), st)
-    rhs = lv0.r.clone()
-    ep = L.ep(L.TG_EP_CHEB_FIRST, rhs=rhs, dinv=lv0.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1, a3=0.5)
-    for _ in range(reps):
-        L.call("tg_apply32", op, L.ptr(rhs), None, ep, st)
+    order = []
+
+    def launches(lv, op, u, y, ep):
+        Lv = g.levels[lv]
+        for _ in range(reps):
+            L.call("tg_apply32", op, L.ptr(u), L.ptr(y), ep, st)
+        order.append({"kernel": f"{kernel_key(lv, op.kind)}/{EP_NAMES[ep.mode]}", "grid_nodes": Lv.nodes,
+                      "algorithmic_bytes": op_bytes(op.kind, Lv.n, Lv.n_elem, ep.mode)})
+
+    for lv in (0, 1, 2, 3):
+        Lv = g.levels[lv]
+        u, r, x, d = Lv.da.clone(), Lv.r.clone(), Lv.x.clone(), torch.zeros_like(Lv.da)
+        rhs, y = Lv.r.clone(), torch.zeros_like(Lv.da)
+        dot = torch.zeros(1, dtype=torch.float64, device=y.device)
+        op = g.op(lv)
+        cheb_r = L.ep(L.TG_EP_CHEB_R, dinv=Lv.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1)
+        if lv == 0:
+            launches(0, op, u, None, cheb_r)
+            launches(0, op, rhs, None, L.ep(L.TG_EP_CHEB_FIRST, rhs=rhs, dinv=Lv.dinv, d=d, x=x, r=r, a1=0.1,
+                                             a2=0.1, a3=0.5))
+            launches(0, op, u, None, L.ep(L.TG_EP_RESID_UPDATE, r=r))
+            launches(0, op, u, None, L.ep(L.TG_EP_CHEB_LAST, dinv=Lv.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1))
+            launches(0, eng.fea.fine_op(), u, y, L.ep(L.TG_EP_DOT, dot=dot))
+            launches(0, op, u, y, L.ep(L.TG_EP_STORE))
+        elif lv == 1:
+            launches(1, op, rhs, None, L.ep(L.TG_EP_RESID_FIRST, rhs=rhs, dinv=Lv.dinv, x=x, r=r, a3=0.5))
+            launches(1, op, u, None, L.ep(L.TG_EP_CHEB_LAST, dinv=Lv.dinv, d=d, x=x, r=r, a1=0.1, a2=0.1))
+        else:
+            launches(lv, op, u, None, cheb_r)
     torch.cuda.synchronize()
-    return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "launches": reps,
-            "kernels": {"k_hex_mfma<false, CHEB_R>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_R),
-                        "k_hex_mfma<true, DOT>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_DOT),
-                        "k_hex_mfma<false, CHEB_FIRST>": hex_bytes(lv0.n, lv0.n_elem, L.TG_EP_CHEB_FIRST)}}
+    return {"copy_bytes": copy_bytes, "copy_kernel": "k_copy32", "launches_each": reps, "order": order}
 
 
 def matvec_bench(reps: int = 200, warmup: int = 5):
@@ -410,8 +424,8 @@ def main() -> int:
     ap.add_argument("--matvec-only", action="store_true", help="only the 128x64x64 matvec microbenchmark")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="two warm-up SIMP iterations, then a 256 MiB device copy and 20 launches each of the "
-                         "level-0 BF16 CHEB_R and CHEB_FIRST smoother steps and the split-BF16 DOT PCG product "
-                         "on the live operands (for hardware-counter runs)")
+                         "level-0, level-1 and stencil-level kernels on the live operands (for hardware-counter "
+                         "runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode; run with --gpus 1", file=sys.stderr)

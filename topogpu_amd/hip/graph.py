@@ -1,22 +1,22 @@
 """Captured launch sequences (tg_graph_* in libtopohip): one Krylov iteration
 of the state solve -- the V-cycle and the PCG updates, about 50 launches --
-captured once and replayed with a single call per iteration.
+captured once as two sequences and replayed with two calls per iteration.
 
 The reference's PCG loop (``_cupy_pcg``, src/gpu_fem/solver_v2.py:552) issues
 every kernel from the host on each iteration. The host cost of a launch
 (argument packing plus the runtime's dispatch, several microseconds) is as
-large as the coarse-level kernels themselves, so a host-driven V-cycle can
-leave the GPU idle between them; a replayed graph cannot.
+large as the coarse-level kernels themselves (about 3.5 us each at
+160x80x80), so a host-driven V-cycle leaves the GPU idle between them after
+every host read of the residual; a replayed graph does not (9.9 against
+10.5 ms per SIMP iteration there). HipStateSolver replays by default
+(graphs=True) and queues the next iteration's first part (the level-0
+pre-smoothing and the restriction, which touch only the hierarchy's own
+buffers) before it waits for r.r.
 
 A graph keeps the arguments its launches had at capture (pointers and
 scalars such as the Chebyshev coefficients), so every graph carries a key of
-what it baked in and is captured again when the key changes.
-
-Measured at 160x80x80 the host keeps ahead of the GPU without graphs (the
-small coarse-level kernels cost about 4 us each on the device, as much as
-their host-side launch), so replay gives the same time per iteration while
-each capture costs about 1.3 ms; HipStateSolver therefore replays only when
-asked (graphs=True), for hosts whose launch path is slower.
+what it baked in and is captured again when the key changes (about 1.3 ms
+per capture, after a lambda re-estimation).
 """
 
 from __future__ import annotations
@@ -32,11 +32,12 @@ from . import lib as L
 class LaunchGraph:
     """An instantiated capture: replay with launch(stream)."""
 
-    def __init__(self, key, handle: int, counts: dict, nbytes: dict):
+    def __init__(self, key, handle: int, counts: dict, nbytes: dict, replay: dict | None = None):
         self.key = key
         self._h = handle
-        self.counts = dict(counts)      # (kernel, epilogue) -> launches per replay (level 0)
+        self.counts = dict(counts)      # (kernel, epilogue) -> accounted launches per replay
         self.nbytes = dict(nbytes)
+        self.replay = dict(replay or {})   # (kernel, epilogue) -> (direct launch, bytes) for timing
 
     def launch(self, stream) -> None:
         L.call("tg_graph_launch", self._h, stream)
@@ -62,10 +63,12 @@ class _Recorder:
     def __init__(self):
         self.counts = defaultdict(int)
         self.bytes = defaultdict(float)
+        self.replay = {}
 
     def launch(self, name, nbytes, fn, mode=-1):
         self.counts[(name, mode)] += 1
         self.bytes[(name, mode)] += float(nbytes)
+        self.replay[(name, mode)] = (fn, float(nbytes))
         fn()
 
 
@@ -92,7 +95,7 @@ def capture(key, issue, owner, device) -> LaunchGraph:
     finally:
         owner.timer = saved
     L.call("tg_graph_end", st, C.byref(h))
-    return LaunchGraph(key, h.value, rec.counts, rec.bytes)
+    return LaunchGraph(key, h.value, rec.counts, rec.bytes, rec.replay)
 
 
 def account(timer, graph: LaunchGraph) -> None:
@@ -102,3 +105,5 @@ def account(timer, graph: LaunchGraph) -> None:
     for k, n in graph.counts.items():
         timer.counts[k] += n
         timer.bytes[k] += graph.nbytes.get(k, 0.0)
+    if hasattr(timer, "replay"):
+        timer.replay.update(graph.replay)

@@ -83,7 +83,7 @@ class HipStateSolver:
                  fine_degree: int = 2, coarse_degree=(1, 4, 6), cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
-                 warm_start: bool = True, level1: str = "q8", graphs: bool = False, speculate: bool = True,
+                 warm_start: bool = True, level1: str = "q8", graphs: bool = True, speculate: bool = True,
                  smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", outer_solver: str = "pcg",
                  restart: int = 32, probe_quality: bool = False, level_precisions=None, device=None):
         if not torch.cuda.is_available():
@@ -156,7 +156,9 @@ class HipStateSolver:
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
-        # one Krylov iteration as a captured launch sequence (first / later)
+        # one Krylov iteration as two captured launch sequences: the V-cycle's
+        # first part ("begin") and the rest with the PCG updates ("finish",
+        # "finish_first")
         self.graphs = bool(graphs)
         self._graphs: dict = {}
         # queue the next V-cycle's first part while the host waits for r.r
@@ -259,7 +261,7 @@ class HipStateSolver:
             while True:
                 # the next V-cycle's first part runs while the host waits for
                 # r.r (wasted, harmlessly, when this was the last iteration)
-                speculate = self.speculate and not self.graphs
+                speculate = self.speculate
                 rr = self._read_rr(st, speculate)
                 if not np.all(np.isfinite(rr)):
                     raise FloatingPointError("non-finite inner residual")
@@ -377,7 +379,10 @@ class HipStateSolver:
             return self.s[2].cpu().numpy()
         self._rr_host.copy_(self.s[2], non_blocking=True)
         self._rr_ready.record()
-        self.gmg.begin(self.r, st)
+        if self._replaying():
+            self._replay("begin", st)
+        else:
+            self.gmg.begin(self.r, st)
         self._rr_ready.synchronize()
         return self._rr_host.numpy().copy()
 
@@ -401,24 +406,38 @@ class HipStateSolver:
         L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
                L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
 
-    def _iteration(self, first: bool, st, begun: bool = False) -> None:
-        """Issue one iteration: replay its captured graph (captured again when
-        what it baked in changed: the GMG's coefficients and buffers, the
-        modulus buffer), or launch it directly when graphs are off or a
-        timer records per-launch events."""
+    def _replaying(self) -> bool:
+        """Graph replay is on and no timer wants per-launch events."""
         timer = self.gmg.timer
-        if begun or not self.graphs or (timer is not None and timer.events):
-            self._issue_iteration(first, st, begun)
-            return
+        return self.graphs and not (timer is not None and timer.events)
+
+    def _replay(self, name: str, st) -> None:
+        """Replay the captured sequence `name` ("begin": the V-cycle's first
+        part; "finish" / "finish_first": the rest of a later / first
+        iteration), capturing it again when what it baked in changed (the
+        hierarchy's coefficients and buffers, the modulus buffer)."""
         key = (self.gmg.graph_key(), L.ptr(self.E32))
-        g = self._graphs.get(first)
+        g = self._graphs.get(name)
         if g is None or g.key != key:
             if g is not None:
                 g.close()
-            g = self._graphs[first] = graph.capture(key, lambda cs: self._issue_iteration(first, cs), self.gmg,
-                                                    self.dev)
+            issue = {"begin": lambda cs: self.gmg.begin(self.r, cs),
+                     "finish": lambda cs: self._issue_iteration(False, cs, begun=True),
+                     "finish_first": lambda cs: self._issue_iteration(True, cs, begun=True)}[name]
+            g = self._graphs[name] = graph.capture(key, issue, self.gmg, self.dev)
         g.launch(st)
-        graph.account(timer, g)
+        graph.account(self.gmg.timer, g)
+
+    def _iteration(self, first: bool, st, begun: bool = False) -> None:
+        """Issue one iteration (whose V-cycle's first part is already queued
+        when begun): two graph replays, or direct launches when graphs are
+        off or a timer records per-launch events."""
+        if not self._replaying():
+            self._issue_iteration(first, st, begun)
+            return
+        if not begun:
+            self._replay("begin", st)
+        self._replay("finish_first" if first else "finish", st)
 
     def compliances(self) -> np.ndarray:
         """F_k . u_k in FP64 for every load case (solver_v4.py:244)."""
