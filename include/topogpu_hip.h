@@ -349,10 +349,12 @@ int tg_compliance_dd(const tg_grid* g, const double* E, const double* u, const d
 
 /* OC bisection of pub_simp_solver.py:319-340 on the device.
  * oc_prepare: q = rho*sqrt(-min(dc,-1e-12)/max(dv,1e-12)),
- *   lo = rho-move, hi = rho+move ; state <- {l1=0, l2=1e9} and the first
+ *   lo = clip(rho-move, 1e-3, 1), hi = clip(rho+move, 1e-3, 1) (clipping
+ *   to [rho-move, rho+move] and then to [1e-3, 1], as the reference does,
+ *   equals one clip to [lo, hi]); state <- {l1=0, l2=1e9} and the first
  *   batch of trial multipliers.
  * oc_bisect_step: one evaluation pass: the filtered+projected volume
- *   sum(heaviside(H clip(q/sqrt(lambda), lo, hi, 1e-3, 1))) at 15 trial
+ *   sum(heaviside(H clip(q/sqrt(lambda), lo, hi))) at 15 trial
  *   multipliers, then the bisection (lm=(l1+l2)/2, l1=lm if volume/n >
  *   volfrac else l2=lm, stop when l2-l1 < 1e-9 or after 100 steps) advances
  *   through every step these volumes decide -- at least two per pass, since

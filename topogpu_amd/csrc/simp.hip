@@ -7,6 +7,8 @@
 // or an output scaling.
 #include <math.h>
 
+#include <type_traits>
+
 #include "common.h"
 #include "reduce.h"
 
@@ -270,10 +272,14 @@ __global__ void k_compliance_dd_finalize(int nb, double* out, Workspace ws) {
   out[3] = e;
 }
 
+// pub_simp_solver.py:326-328 clips to [rho-move, rho+move], then to
+// [1e-3, 1]. For lo <= hi that is one clip to [lo', hi'] with lo', hi' the
+// ends clipped to [1e-3, 1] (clips are monotone), which k_oc_prepare stores.
+__device__ __forceinline__ double clip_bounds(double v) { return fmin(fmax(v, 1e-3), 1.0); }
+
 __device__ __forceinline__ double oc_candidate(double q, double lo, double hi, double lm) {
-  // pub_simp_solver.py:326-328: clip to [rho-move, rho+move], then [1e-3, 1]
   const double x = q * (1.0 / sqrt(lm));   // same arithmetic as k_oc_pass
-  return fmin(fmax(fmin(fmax(x, lo), hi), 1e-3), 1.0);
+  return fmin(fmax(x, lo), hi);
 }
 
 // OC bisection state (device doubles).
@@ -305,8 +311,8 @@ __global__ void k_oc_prepare(long long n, const double* __restrict__ rho, const 
   if (e >= n) return;
   const double dcs = fmin(dc[e], -1e-12), dvs = fmax(dv[e], 1e-12);
   q[e] = rho[e] * sqrt(-dcs / dvs);
-  lo[e] = rho[e] - move;
-  hi[e] = rho[e] + move;
+  lo[e] = clip_bounds(rho[e] - move);
+  hi[e] = clip_bounds(rho[e] + move);
 }
 
 // Volume sum(heaviside(H x(lambda))) at the OC_NPTS multipliers of the state,
@@ -338,7 +344,7 @@ __global__ void __launch_bounds__(256) k_oc_pass(Grid g, Stencil st, const doubl
       const double qf = q[f], lf = lo[f], hf = hi[f], w = st.w[o];
 #pragma unroll
       for (int t = 0; t < OC_NPTS; ++t)
-        s[t] = fma(w, fmin(fmax(fmin(fmax(qf * rs[t], lf), hf), 1e-3), 1.0), s[t]);
+        s[t] = fma(w, fmin(fmax(qf * rs[t], lf), hf), s[t]);
     }
     const double hinv = hs_inv[e];
 #pragma unroll
@@ -350,164 +356,227 @@ __global__ void __launch_bounds__(256) k_oc_pass(Grid g, Stencil st, const doubl
   block_partials<OC_NPTS>(v, ws);
 }
 
-// Tiled form of k_oc_pass for stencils of radius <= OC_R. A block owns an
-// OTX x OTY x OTZ element tile; for each group of OC_G multipliers it writes
-// the clipped candidate densities of the tile and its halo to LDS once (one
-// clip per element and multiplier instead of one per stencil neighbour),
-// then every thread filters and projects a z-run of OC_RUN elements from LDS.
-constexpr int OTX = 8, OTY = 8, OTZ = 16, OC_THREADS = 256, OC_RUN = OTX * OTY * OTZ / OC_THREADS;
-constexpr int OC_R = 1, OC_G = 3;
+// Tiled forms of k_stencil and k_oc_pass for radius-1 stencils whose offsets
+// come as filter_stencil lists them: the element itself first, then the other
+// offsets in increasing lexicographic (dx, dy, dz) order. A block owns a
+// TX x TY x TZ element tile and stages the tile and its one-element halo in
+// LDS (zero outside the grid, which adds w * 0 like the skipped neighbour of
+// k_stencil); each thread then sums a z-run of T_RUN elements from LDS, one
+// (dx, dy) column of T_RUN + 2 values at a time read as 16-byte words. Every
+// output adds its terms in the stencil's order, so for finite inputs the
+// result is that of k_stencil bit for bit.
+constexpr int TX = 8, TY = 8, TZ = 16, T_THREADS = 256, T_RUN = TX * TY * TZ / T_THREADS;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, T_NH = HX * HY * HZ;
+constexpr int T_HIT = (T_NH + T_THREADS - 1) / T_THREADS;
+constexpr int OC_G = 3, OC_NG = OC_NPTS / OC_G;   // multipliers per block, blocks per tile
+constexpr int kXcds = 8;
 static_assert(OC_NPTS % OC_G == 0, "multiplier groups");
-static_assert(OTZ % OC_RUN == 0, "z-runs tile the z extent");
+static_assert(TZ % T_RUN == 0 && T_RUN % 2 == 0 && HZ % 2 == 0, "z-runs of 16-byte aligned columns");
 
-__global__ void __launch_bounds__(OC_THREADS) k_oc_pass_tiled(Grid g, Stencil st, int R, const double* __restrict__ hs_inv,
-                                                             const double* __restrict__ q,
-                                                             const double* __restrict__ lo,
-                                                             const double* __restrict__ hi, double beta, double eta,
-                                                             int use_h, const double* state, Workspace ws) {
-  if (state[S_DONE] != 0.0) return;   // uniform: converged bracket
-  constexpr int HMAX = (OTX + 2 * OC_R) * (OTY + 2 * OC_R) * (OTZ + 2 * OC_R);
-  constexpr int HIT = (HMAX + OC_THREADS - 1) / OC_THREADS;
-  __shared__ double sx[OC_G][HMAX];
-  const int HX = OTX + 2 * R, HY = OTY + 2 * R, HZ = OTZ + 2 * R, NH = HX * HY * HZ;
-  const int ntz = (g.nz + OTZ - 1) / OTZ, nty = (g.ny + OTY - 1) / OTY;
-  const int bt = blockIdx.x;
-  const int k0 = (bt % ntz) * OTZ, j0 = ((bt / ntz) % nty) * OTY, i0 = (bt / (ntz * nty)) * OTX;
+// w[(dx + 1) * 9 + (dy + 1) * 3 + dz + 1], zero where the stencil holds no
+// offset; bit t of mask set where it does
+struct Cube27 {
+  double w[27];
+  unsigned mask;
+};
 
-  const bool proj_on = use_h && beta > 1e-6;
-  const double p0 = tanh_d(beta * eta), ipden = 1.0 / (p0 + tanh_d(beta * (1.0 - eta)));
-  // this thread's z-run
-  const int tz = threadIdx.x % (OTZ / OC_RUN), ty = (threadIdx.x / (OTZ / OC_RUN)) % OTY,
-            tx = threadIdx.x / ((OTZ / OC_RUN) * OTY);
-  const int oi = i0 + tx, oj = j0 + ty, ok0 = k0 + tz * OC_RUN;
-  double hinv[OC_RUN];
-  bool own[OC_RUN];
-#pragma unroll
-  for (int r = 0; r < OC_RUN; ++r) {
-    own[r] = oi < g.nx && oj < g.ny && ok0 + r < g.nz;
-    hinv[r] = own[r] ? hs_inv[g.elem(oi, oj, ok0 + r)] : 0.0;
+bool make_cube27(const Stencil& st, Cube27& c) {
+  if (st.off[0][0] != 0 || st.off[0][1] != 0 || st.off[0][2] != 0) return false;
+  c.mask = 1u << 13;
+  for (int t = 0; t < 27; ++t) c.w[t] = 0.0;
+  c.w[13] = st.w[0];
+  int last = -1;
+  for (int k = 1; k < st.m; ++k) {
+    const int dx = st.off[k][0], dy = st.off[k][1], dz = st.off[k][2];
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return false;
+    const int t = (dx + 1) * 9 + (dy + 1) * 3 + dz + 1;
+    if (t <= last || t == 13) return false;
+    last = t;
+    c.w[t] = st.w[k];
+    c.mask |= 1u << t;
   }
-  const int obase = ((tx + R) * HY + (ty + R)) * HZ + tz * OC_RUN + R;
+  return true;
+}
 
-  const int nb = gridDim.x;
-  // one group of OC_G multipliers per block row: the tiles' groups run as
-  // independent workgroups, which spreads the pass over many more of them
-  {
-    const int grp = blockIdx.y;
-    double rs[OC_G];
+struct Tile {
+  int i0, j0, k0;
+};
+
+__device__ __forceinline__ Tile tile_origin(const Grid& g, int t) {
+  const int ntz = (g.nz + TZ - 1) / TZ, nty = (g.ny + TY - 1) / TY;
+  return {(t / (ntz * nty)) * TX, ((t / ntz) % nty) * TY, (t % ntz) * TZ};
+}
+
+// halo slot idx -> element index f of the grid; false outside it (the tiled
+// kernels run on grids of fewer than 2^31 elements: 32-bit index arithmetic)
+__device__ __forceinline__ bool halo_elem(const Grid& g, const Tile& tl, int idx, int& f) {
+  const int hz = idx % HZ, hy = (idx / HZ) % HY, hx = idx / (HZ * HY);
+  const int i = tl.i0 - 1 + hx, j = tl.j0 - 1 + hy, k = tl.k0 - 1 + hz;
+  // unsigned compares test both ends; & rather than && keeps it branch-free
+  const bool in = (idx < T_NH) & ((unsigned)i < (unsigned)g.nx) & ((unsigned)j < (unsigned)g.ny) &
+                  ((unsigned)k < (unsigned)g.nz);
+  f = in ? (i * g.ny + j) * g.nz + k : 0;
+  return in;
+}
+
+__device__ __forceinline__ void load_col(const double* s, double (&x)[T_RUN + 2]) {
+  const double2* p = reinterpret_cast<const double2*>(s);
 #pragma unroll
-    for (int c = 0; c < OC_G; ++c) rs[c] = 1.0 / sqrt(state[S_CAND + grp * OC_G + c]);
-    // halo inputs (re-read per group from L2; out-of-grid: no density)
-#pragma unroll
-    for (int it = 0; it < HIT; ++it) {
-      const int idx = threadIdx.x + it * OC_THREADS;
-      const int hz = idx % HZ, hy = (idx / HZ) % HY, hx = idx / (HZ * HY);
-      const int i = i0 - R + hx, j = j0 - R + hy, k = k0 - R + hz;
-      const bool in = idx < NH && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
-      const long long f = in ? g.elem(i, j, k) : 0;
-      const double hq = in ? q[f] : 0.0, hl = in ? lo[f] : 0.0, hh = in ? hi[f] : 0.0;
-      if (idx < NH)
-#pragma unroll
-        for (int c = 0; c < OC_G; ++c)
-          sx[c][idx] = in ? fmin(fmax(fmin(fmax(hq * rs[c], hl), hh), 1e-3), 1.0) : 0.0;
-    }
-    __syncthreads();
-    double acc[OC_G][OC_RUN];
-#pragma unroll
-    for (int c = 0; c < OC_G; ++c)
-#pragma unroll
-      for (int r = 0; r < OC_RUN; ++r) acc[c][r] = 0.0;
-    for (int o = 0; o < st.m; ++o) {
-      const int off = obase + (st.off[o][0] * HY + st.off[o][1]) * HZ + st.off[o][2];
-      const double w = st.w[o];
-#pragma unroll
-      for (int c = 0; c < OC_G; ++c)
-#pragma unroll
-        for (int r = 0; r < OC_RUN; ++r) acc[c][r] = fma(w, sx[c][off + r], acc[c][r]);
-    }
-    double vg[OC_G];
-#pragma unroll
-    for (int c = 0; c < OC_G; ++c) {
-      vg[c] = 0.0;
-#pragma unroll
-      for (int r = 0; r < OC_RUN; ++r)
-        if (own[r]) {
-          const double rf = acc[c][r] * hinv[r];
-          vg[c] += proj_on ? (p0 + tanh_d(beta * (rf - eta))) * ipden : rf;
-        }
-    }
-    // partials in the layout of block_partials<OC_NPTS>
-    block_sum<OC_G>(vg);
-    if (threadIdx.x == 0)
-#pragma unroll
-      for (int c = 0; c < OC_G; ++c) ws.partials[(size_t)(grp * OC_G + c) * nb + blockIdx.x] = vg[c];
+  for (int u = 0; u < (T_RUN + 2) / 2; ++u) {
+    const double2 v = p[u];
+    x[2 * u] = v.x;
+    x[2 * u + 1] = v.y;
   }
 }
 
-// One block: sums the pass's volumes, then
-//  1. tightens the bracket [a, b] of the volume crossing;
-//  2. advances the bisection of pub_simp_solver.py:333-338 (lm = (l1+l2)/2,
-//     l1 = lm when volume/n > volfrac else l2 = lm, stop when l2-l1 < 1e-9
-//     or after max_steps) through every step the evaluations decide: a
-//     midpoint that was itself evaluated uses its own volume, exactly like
-//     the sequential loop; a midpoint at or below an evaluated multiplier
-//     whose volume is above the target (at or above one whose volume is at
-//     or below it) takes that decision, which the monotonicity of the volume
-//     in lambda implies. Only evaluations whose volume clears the target by
-//     more than any rounding-level non-monotonicity could bridge are used
-//     this way (sa, sb);
-//  3. chooses the next pass's multipliers: the three midpoints of the next
-//     two bisection levels (exact decisions, so every pass advances at least
-//     two steps), one third-level midpoint, and a secant prediction of the
-//     crossing in s = lambda^(-1/2) (the unclipped volume is linear in s)
-//     with ten geometrically spaced neighbours, which shrink [a, b] around
-//     the crossing so that whole runs of later midpoints are decided. Once
-//     [a, b] is narrower than the bisection's final resolution, the pass
-//     evaluates the 15 midpoints of the next four levels instead.
-// The filter on OTX x OTY x OTZ element tiles: the (scaled) input of the
-// tile and its R-element halo is staged in LDS once, then each thread sums a
-// z-run of OC_RUN elements. Terms are added in the stencil's order and an
-// out-of-grid neighbour adds w * 0, so the result is that of k_stencil.
-__global__ void __launch_bounds__(OC_THREADS) k_stencil_tiled(Grid g, Stencil st, int R, const double* __restrict__ in,
-                                                             const double* __restrict__ in_scale,
-                                                             const double* __restrict__ out_scale,
-                                                             double* __restrict__ out) {
-  constexpr int HMAX = (OTX + 2 * OC_R) * (OTY + 2 * OC_R) * (OTZ + 2 * OC_R);
-  constexpr int HIT = (HMAX + OC_THREADS - 1) / OC_THREADS;
-  __shared__ double sx[HMAX];
-  const int HX = OTX + 2 * R, HY = OTY + 2 * R, HZ = OTZ + 2 * R, NH = HX * HY * HZ;
-  const int ntz = (g.nz + OTZ - 1) / OTZ, nty = (g.ny + OTY - 1) / OTY;
-  const int bt = blockIdx.x;
-  const int k0 = (bt % ntz) * OTZ, j0 = ((bt / ntz) % nty) * OTY, i0 = (bt / (ntz * nty)) * OTX;
+// acc[c][r] = the stencil sum for the thread's r-th element from slab c of
+// sx, over the offsets of MASK (a compile-time set, so the sum is straight
+// FMAs: a run-time mask would be if-converted into selects)
+template <int G, unsigned MASK>
+__device__ __forceinline__ void cube27_run(const Cube27& cw, const double (*sx)[T_NH], int tx, int ty, int tz,
+                                           double (&acc)[G][T_RUN]) {
+  const int base = tz * T_RUN;
 #pragma unroll
-  for (int it = 0; it < HIT; ++it) {
-    const int idx = threadIdx.x + it * OC_THREADS;
-    const int hz = idx % HZ, hy = (idx / HZ) % HY, hx = idx / (HZ * HY);
-    const int i = i0 - R + hx, j = j0 - R + hy, k = k0 - R + hz;
-    const bool in_g = idx < NH && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
-    const long long f = in_g ? g.elem(i, j, k) : 0;
-    const double v = in_g ? (in_scale ? in[f] * in_scale[f] : in[f]) : 0.0;
-    if (idx < NH) sx[idx] = v;
+  for (int c = 0; c < G; ++c) {
+    double x[T_RUN + 2];
+    load_col(&sx[c][((tx + 1) * HY + ty + 1) * HZ + base], x);
+#pragma unroll
+    for (int r = 0; r < T_RUN; ++r) acc[c][r] = cw.w[13] * x[r + 1];
+  }
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const int t0 = dx * 9 + dy * 3;
+      constexpr unsigned kAll = 7u;
+      const unsigned m = (MASK >> t0) & (dx == 1 && dy == 1 ? 5u : kAll);
+      if (m == 0) continue;
+      const double w0 = cw.w[t0], w1 = cw.w[t0 + 1], w2 = cw.w[t0 + 2];
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        double x[T_RUN + 2];
+        load_col(&sx[c][((tx + dx) * HY + ty + dy) * HZ + base], x);
+#pragma unroll
+        for (int r = 0; r < T_RUN; ++r) {
+          if (m & 1u) acc[c][r] = fma(w0, x[r], acc[c][r]);
+          if (m & 2u) acc[c][r] = fma(w1, x[r + 1], acc[c][r]);
+          if (m & 4u) acc[c][r] = fma(w2, x[r + 2], acc[c][r]);
+        }
+      }
+    }
+}
+
+// The offset sets of the radius-1 cone filter (filter_stencil): the element
+// alone (rmin <= 1), with its 6 face neighbours (rmin <= sqrt 2), with the
+// 12 edge neighbours too (rmin <= sqrt 3), the whole cube. Another set runs
+// as the whole cube with zero weights, whose terms 0 * x leave a sum of
+// finite values unchanged.
+constexpr unsigned cube_mask(int max_d2) {
+  unsigned m = 0;
+  for (int t = 0; t < 27; ++t) {
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    if (dx * dx + dy * dy + dz * dz <= max_d2) m |= 1u << t;
+  }
+  return m;
+}
+constexpr unsigned kCube1 = cube_mask(0), kCube7 = cube_mask(1), kCube19 = cube_mask(2), kCube27 = cube_mask(3);
+
+// fn(std::integral_constant<unsigned, MASK>) for the kernel instance of mask
+template <class F>
+void with_cube_mask(unsigned mask, F&& fn) {
+  if (mask == kCube19) fn(std::integral_constant<unsigned, kCube19>());
+  else if (mask == kCube7) fn(std::integral_constant<unsigned, kCube7>());
+  else if (mask == kCube1) fn(std::integral_constant<unsigned, kCube1>());
+  else fn(std::integral_constant<unsigned, kCube27>());
+}
+
+// The OC pass on tiles: OC_NPTS / OC_G blocks per tile, each evaluating
+// OC_G of the pass's multipliers (it clips the tile's candidate densities for
+// them into LDS once, instead of once per stencil neighbour). A tile's blocks
+// are numbered kXcds apart, so that they run back to back on one XCD, where
+// all but the first read the tile's inputs from that XCD's L2.
+template <unsigned MASK>
+__global__ void __launch_bounds__(T_THREADS) k_oc_pass_tiled(Grid g, Cube27 cw, int nb, const double* __restrict__ hs_inv, const double* __restrict__ q,
+                const double* __restrict__ lo, const double* __restrict__ hi, double beta, double eta, int use_h,
+                const double* state, Workspace ws) {
+  if (state[S_DONE] != 0.0) return;   // uniform: converged bracket
+  const int s = blockIdx.x / kXcds, grp = s % OC_NG, t = (s / OC_NG) * kXcds + blockIdx.x % kXcds;
+  if (t >= nb) return;   // uniform: padding of the last round of kXcds tiles
+  __shared__ __align__(16) double sx[OC_G][T_NH];
+  const Tile tl = tile_origin(g, t);
+  double rs[OC_G];
+#pragma unroll
+  for (int c = 0; c < OC_G; ++c) rs[c] = 1.0 / sqrt(state[S_CAND + grp * OC_G + c]);
+#pragma unroll
+  for (int it = 0; it < T_HIT; ++it) {
+    const int idx = threadIdx.x + it * T_THREADS;
+    int f;
+    const bool in = halo_elem(g, tl, idx, f);
+    const double hq = q[f], hl = lo[f], hh = hi[f];   // element 0 stands in outside the grid
+    if (idx < T_NH)
+#pragma unroll
+      for (int c = 0; c < OC_G; ++c) sx[c][idx] = in ? fmin(fmax(hq * rs[c], hl), hh) : 0.0;
+  }
+  const int tz = threadIdx.x % (TZ / T_RUN), ty = (threadIdx.x / (TZ / T_RUN)) % TY,
+            tx = threadIdx.x / ((TZ / T_RUN) * TY);
+  const int oi = tl.i0 + tx, oj = tl.j0 + ty, ok0 = tl.k0 + tz * T_RUN;
+  double hinv[T_RUN];
+#pragma unroll
+  for (int r = 0; r < T_RUN; ++r)
+    hinv[r] = oi < g.nx && oj < g.ny && ok0 + r < g.nz ? hs_inv[g.elem(oi, oj, ok0 + r)] : 0.0;
+  __syncthreads();
+  const bool proj_on = use_h && beta > 1e-6;
+  const double p0 = tanh_d(beta * eta), ipden = 1.0 / (p0 + tanh_d(beta * (1.0 - eta)));
+  double vg[OC_G];
+#pragma unroll
+  for (int c = 0; c < OC_G; ++c) {
+    // one multiplier at a time: its sums are projected before the next one's
+    // LDS reads are issued (sched_barrier), which bounds the live registers
+    double acc[1][T_RUN];
+    cube27_run<1, MASK>(cw, &sx[c], tx, ty, tz, acc);
+    vg[c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < T_RUN; ++r)
+      if (oi < g.nx && oj < g.ny && ok0 + r < g.nz) {
+        const double rf = acc[0][r] * hinv[r];
+        vg[c] += proj_on ? (p0 + tanh_d(beta * (rf - eta))) * ipden : rf;
+      }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // partials in the layout of block_partials<OC_NPTS>
+  block_sum<OC_G>(vg);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < OC_G; ++c) ws.partials[(size_t)(grp * OC_G + c) * nb + t] = vg[c];
+}
+
+template <unsigned MASK>
+__global__ void __launch_bounds__(T_THREADS) k_stencil_tiled(Grid g, Cube27 cw, const double* __restrict__ in,
+                                                            const double* __restrict__ in_scale,
+                                                            const double* __restrict__ out_scale,
+                                                            double* __restrict__ out) {
+  __shared__ double sx[1][T_NH];
+  const Tile tl = tile_origin(g, blockIdx.x);
+#pragma unroll
+  for (int it = 0; it < T_HIT; ++it) {
+    const int idx = threadIdx.x + it * T_THREADS;
+    int f;
+    const bool in_g = halo_elem(g, tl, idx, f);
+    const double x = in_scale ? in[f] * in_scale[f] : in[f];   // element 0 stands in outside the grid
+    const double v = in_g ? x : 0.0;
+    if (idx < T_NH) sx[0][idx] = v;
   }
   __syncthreads();
-  const int tz = threadIdx.x % (OTZ / OC_RUN), ty = (threadIdx.x / (OTZ / OC_RUN)) % OTY,
-            tx = threadIdx.x / ((OTZ / OC_RUN) * OTY);
-  const int oi = i0 + tx, oj = j0 + ty, ok0 = k0 + tz * OC_RUN;
-  const int obase = ((tx + R) * HY + (ty + R)) * HZ + tz * OC_RUN + R;
-  double acc[OC_RUN];
+  const int tz = threadIdx.x % (TZ / T_RUN), ty = (threadIdx.x / (TZ / T_RUN)) % TY,
+            tx = threadIdx.x / ((TZ / T_RUN) * TY);
+  const int oi = tl.i0 + tx, oj = tl.j0 + ty, ok0 = tl.k0 + tz * T_RUN;
+  double acc[1][T_RUN];
+  cube27_run<1, MASK>(cw, sx, tx, ty, tz, acc);
 #pragma unroll
-  for (int r = 0; r < OC_RUN; ++r) acc[r] = 0.0;
-  for (int o = 0; o < st.m; ++o) {
-    const int off = obase + (st.off[o][0] * HY + st.off[o][1]) * HZ + st.off[o][2];
-    const double w = st.w[o];
-#pragma unroll
-    for (int r = 0; r < OC_RUN; ++r) acc[r] = fma(w, sx[off + r], acc[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < OC_RUN; ++r) {
+  for (int r = 0; r < T_RUN; ++r) {
     if (oi < g.nx && oj < g.ny && ok0 + r < g.nz) {
       const long long e = g.elem(oi, oj, ok0 + r);
-      out[e] = out_scale ? acc[r] * out_scale[e] : acc[r];
+      out[e] = out_scale ? acc[0][r] * out_scale[e] : acc[0][r];
     }
   }
 }
@@ -542,6 +611,26 @@ __device__ void sum_candidate_partials(const double* partials, int nb, int nc, d
   __syncthreads();
 }
 
+// One block: sums the pass's volumes, then
+//  1. tightens the bracket [a, b] of the volume crossing;
+//  2. advances the bisection of pub_simp_solver.py:333-338 (lm = (l1+l2)/2,
+//     l1 = lm when volume/n > volfrac else l2 = lm, stop when l2-l1 < 1e-9
+//     or after max_steps) through every step the evaluations decide: a
+//     midpoint that was itself evaluated uses its own volume, exactly like
+//     the sequential loop; a midpoint at or below an evaluated multiplier
+//     whose volume is above the target (at or above one whose volume is at
+//     or below it) takes that decision, which the monotonicity of the volume
+//     in lambda implies. Only evaluations whose volume clears the target by
+//     more than any rounding-level non-monotonicity could bridge are used
+//     this way (sa, sb);
+//  3. chooses the next pass's multipliers: the three midpoints of the next
+//     two bisection levels (exact decisions, so every pass advances at least
+//     two steps), one third-level midpoint, and a secant prediction of the
+//     crossing in s = lambda^(-1/2) (the unclipped volume is linear in s)
+//     with ten geometrically spaced neighbours, which shrink [a, b] around
+//     the crossing so that whole runs of later midpoints are decided. Once
+//     [a, b] is narrower than the bisection's final resolution, the pass
+//     evaluates the 15 midpoints of the next four levels instead.
 __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double volfrac, int max_steps, double* state,
                                                 Workspace ws) {
   if (state[S_DONE] != 0.0) return;
@@ -863,15 +952,14 @@ extern "C" int tg_stencil64(const tg_grid* gp, int32_t m, const int32_t* offs, c
   Stencil st;
   if (int r = make_stencil(m, offs, w, st)) return r;
   const Grid g = to_grid(gp);
-  int R = 0;
-  for (int k = 0; k < st.m; ++k)
-    for (int d = 0; d < 3; ++d) R = abs(st.off[k][d]) > R ? abs(st.off[k][d]) : R;
-  if (R <= OC_R) {
-    const long long tiles =
-        (long long)((g.nx + OTX - 1) / OTX) * ((g.ny + OTY - 1) / OTY) * ((g.nz + OTZ - 1) / OTZ);
+  Cube27 cw;
+  if (make_cube27(st, cw) && g.elems() < (1LL << 31)) {
+    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + TZ - 1) / TZ);
     TG_REQUIRE(tiles < (1LL << 31), "stencil64: grid too large");
-    hipLaunchKernelGGL(k_stencil_tiled, dim3((unsigned)tiles), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R, in,
-                       in_scale, out_scale, out);
+    with_cube_mask(cw.mask, [&](auto mk) {
+      hipLaunchKernelGGL(k_stencil_tiled<decltype(mk)::value>, dim3((unsigned)tiles), dim3(T_THREADS), 0,
+                         (hipStream_t)s, g, cw, in, in_scale, out_scale, out);
+    });
   } else {
     hipLaunchKernelGGL(k_stencil, dim3((unsigned)((g.elems() + 255) / 256)), dim3(256), 0, (hipStream_t)s, g, st,
                        in, in_scale, out_scale, out);
@@ -956,17 +1044,17 @@ extern "C" int tg_oc_bisect_step(const tg_grid* gp, int32_t m, const int32_t* of
   Stencil st;
   if (int r = make_stencil(m, offs, w, st)) return r;
   const Grid g = to_grid(gp);
-  int R = 0;
-  for (int k = 0; k < st.m; ++k)
-    for (int d = 0; d < 3; ++d) R = st.off[k][d] > R ? st.off[k][d] : (-st.off[k][d] > R ? -st.off[k][d] : R);
+  Cube27 cw;
   int nb;
-  if (R <= OC_R) {
-    const long long tiles = (long long)((g.nx + OTX - 1) / OTX) * ((g.ny + OTY - 1) / OTY) * ((g.nz + OTZ - 1) / OTZ);
+  if (make_cube27(st, cw) && g.elems() < (1LL << 31)) {
+    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + TZ - 1) / TZ);
     TG_REQUIRE(tiles * OC_NPTS <= kMaxPartials, "grid too large for the reduction workspace");
     nb = (int)tiles;
-    hipLaunchKernelGGL(k_oc_pass_tiled, dim3(nb, OC_NPTS / OC_G), dim3(OC_THREADS), 0, (hipStream_t)s, g, st, R,
-                       hs_inv, q, lo, hi,
-                       beta, eta, use_h, state, workspace());
+    const unsigned blocks = (unsigned)((nb + kXcds - 1) / kXcds) * kXcds * OC_NG;
+    with_cube_mask(cw.mask, [&](auto mk) {
+      hipLaunchKernelGGL(k_oc_pass_tiled<decltype(mk)::value>, dim3(blocks), dim3(T_THREADS), 0, (hipStream_t)s, g,
+                         cw, nb, hs_inv, q, lo, hi, beta, eta, use_h, state, workspace());
+    });
   } else {
     nb = grid_blocks(g.elems());
     hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(256), 0, (hipStream_t)s, g, st, hs_inv, q, lo, hi, beta, eta,
