@@ -248,6 +248,25 @@ int tg_dense_spd_inverse(double* A, int32_t n, void* stream);
 int tg_coarsest_apply(const double* Ainv, int32_t n_free, const int32_t* free_list,
                       int64_t n_full, const float* rc, float* ec, void* stream);
 
+/* Coarsest solve by block cyclic reduction over the x-planes of the level
+ * (each plane's 3*(nely+1)*(nelz+1) DOFs one block, at most 2048): an exact
+ * FP64 block Cholesky factorisation of the block-tridiagonal operator in
+ * odd-even plane order, for coarsest levels too large for the dense inverse.
+ * Replaces cp.linalg.cholesky + solve_triangular (multigrid_v4.py:781-801,
+ * 977-978) the way tg_dense_spd_inverse + tg_coarsest_apply do, with a
+ * setup of batched plane-block inverses and DGEMMs per reduction level and
+ * a solve of one GEMV kernel per level and direction.
+ * bcr_size: doubles of the factor buffer (work) for grid g.
+ * bcr_setup: assembles the blocks from the level's element matrices (mask
+ *   as for the other level operators) and factors them into work.
+ *   Host-synchronous; TG_ERR_NOT_SPD when a plane block is not SPD.
+ * bcr_apply_b: e_coarse = A^-1 r_coarse for nrhs full-length slices (fixed
+ *   DOFs come out 0); vec: 2 * nrhs * 3 * nodes FP64 scratch. */
+int tg_bcr_size(const tg_grid* g, int64_t* n_work);
+int tg_bcr_setup(const tg_grid* g, const uint8_t* mask, const double* mats64, double* work, void* stream);
+int tg_bcr_apply_b(const tg_grid* g, const double* work, int32_t nrhs, const float* r_coarse,
+                   float* e_coarse, double* vec, void* stream);
+
 /* ---- Krylov vector kernels (FP32 vectors, FP64 reductions) -------------- */
 
 /* *out = sum(a*b) (and *out2 = sum(a*c) when c != NULL), FP64 accumulate.

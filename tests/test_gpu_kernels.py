@@ -8,6 +8,8 @@ rounded to FP32 first so that only kernel arithmetic is measured.
 
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -424,6 +426,65 @@ def test_galerkin_levels_match_ptkp(dims, point):
             e = h(ec)
             assert rel_l2(e[fr], np.linalg.solve(Kd, rc[fr].astype(float))) < 1e-6
             assert np.all(e[np.setdiff1d(np.arange(nd), fr)] == 0.0)
+
+
+def _bcr_solve(dims, mask, m64, rc):
+    """tg_bcr_setup + tg_bcr_apply_b on nrhs = len(rc) right-hand sides:
+    the FP32 output and the FP64 solution kept in the scratch vector."""
+    nw = C.c_int64(0)
+    L.call("tg_bcr_size", L.grid(dims), C.byref(nw))
+    work = torch.zeros(int(nw.value), dtype=torch.float64, device=DEV)
+    L.call("tg_bcr_setup", L.grid(dims), L.ptr(t(mask)), L.ptr(m64), L.ptr(work), stream())
+    nrhs, nd = rc.shape
+    ec = torch.full((nrhs * nd,), np.nan, dtype=torch.float32, device=DEV)
+    vec = torch.full((2 * nrhs * nd,), np.nan, dtype=torch.float64, device=DEV)
+    L.call("tg_bcr_apply_b", L.grid(dims), L.ptr(work), nrhs, L.ptr(t(rc.ravel())), L.ptr(ec), L.ptr(vec),
+           stream())
+    return h(ec).reshape(nrhs, nd), h(vec).reshape(nrhs, 2, nd)[:, 1]
+
+
+# plane counts 2, 3, 6, 7, 21: odd and even active counts at every level,
+# the last odd plane with and without a right neighbour; interior pinned
+# nodes put fixed DOFs inside planes; (20, 10, 10) is the coarsest level of
+# the 4-level 160x80x80 hierarchy (363-DOF planes)
+@pytest.mark.parametrize("dims,point", [((1, 2, 2), False), ((2, 3, 1), True), ((5, 2, 2), False),
+                                        ((6, 4, 3), True), ((20, 3, 2), True), ((20, 10, 10), False)])
+def test_block_cyclic_reduction_solves_coarsest(dims, point):
+    """The plane-block cyclic reduction solve equals a direct solve of the
+    assembled free-DOF matrix (backward error at FP64 rounding level), for
+    two right-hand sides in one launch, with fixed DOFs returned as zero."""
+    import scipy.sparse.linalg as spla
+    free, F, ndof, E, mask, rng = _setup(dims, 4, point_support=point)
+    m64 = t((E[:, None, None] * fem.unit_stiffness()[None]).ravel())
+    K = oracle.assemble_k(oracle.edof_table_3d(*dims), E, ndof)[free][:, free].tocsc()
+    # the element matrices are the oracle's K, entry by entry (dense assembly)
+    if free.size <= 2048:
+        fidx = -np.ones(ndof, np.int32)
+        fidx[free] = np.arange(free.size, dtype=np.int32)
+        A = torch.zeros(free.size * free.size, dtype=torch.float64, device=DEV)
+        L.call("tg_assemble_dense", L.grid(dims), L.ptr(m64), L.ptr(t(fidx)), free.size, L.ptr(A), stream())
+        Kd = K.toarray()
+        assert np.abs(h(A).reshape(Kd.shape) - Kd).max() < 1e-12 * np.abs(Kd).max()
+    rc = np.zeros((2, ndof), np.float32)
+    rc[:, free] = rng.standard_normal((2, free.size))
+    e32, x64 = _bcr_solve(dims, mask, m64, rc)
+    fixed = np.setdiff1d(np.arange(ndof), free)
+    knorm = abs(K).sum(axis=0).max()
+    for k in range(2):
+        b = rc[k, free].astype(np.float64)
+        x = x64[k, free]
+        berr = np.linalg.norm(K @ x - b) / (knorm * np.linalg.norm(x) + np.linalg.norm(b))
+        assert berr < 1e-13, berr
+        assert rel_l2(e32[k, free], spla.spsolve(K, b)) < 1e-6
+        assert np.all(e32[k, fixed] == 0.0) and np.all(x64[k, fixed] == 0.0)
+
+
+def test_block_cyclic_reduction_reports_not_spd():
+    dims = (5, 2, 2)
+    free, F, ndof, E, mask, rng = _setup(dims, 4)
+    m64 = t(np.zeros(576 * int(np.prod(dims))))   # free rows with a zero diagonal
+    with pytest.raises(L.TopoHipError, match="not SPD"):
+        _bcr_solve(dims, mask, m64, np.zeros((1, ndof), np.float32))
 
 
 @pytest.mark.parametrize("dims", [(8, 4, 4), (16, 8, 12)])

@@ -11,6 +11,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include "common.h"
+#include "dense.h"
 #include "reduce.h"
 
 namespace tg {
@@ -248,7 +249,11 @@ __global__ void __launch_bounds__(256) k_assemble_dense(Grid g, const double* __
 }
 
 // A (n x n, upper triangle of the row-major array valid) -> full symmetric.
-__global__ void k_symmetrize_from_upper(double* A, int n) {
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// Matrix blockIdx.y of a batch at stride sa.
+__global__ void k_symmetrize_from_upper(double* A, int n, long long sa) {
+  A += blockIdx.y * sa;
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= (long long)n * n) return;
   const int r = (int)(t / n), c = (int)(t % n);
@@ -298,10 +303,14 @@ constexpr int BK = 16, TS = 64;
 
 // Invert the b x b matrix at A (leading dimension lda) into P (leading
 // dimension ldp; P may be A). info: first failing pivot + base + 1; with
-// zero_info, thread 0 clears *info first.
+// zero_info, thread 0 clears *info first. Block x of a batch works on the
+// matrices at A + x * sa and P + x * sp.
 template <int NB>
 __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(const double* A, int lda, int n, double* P, int ldp,
-                                                             int* info, int info_base, int zero_info) {
+                                                             int* info, int info_base, int zero_info, long long sa,
+                                                             long long sp) {
+  A += blockIdx.x * sa;
+  P += blockIdx.x * sp;
   __shared__ double prow[32 * NB];
   __shared__ double pcol[32 * NB];
   const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
@@ -368,8 +377,12 @@ __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_gj(const double* A, 
 }
 
 // A_kj <- P A_kj: rows [k0, k0 + kw) of column tile blockIdx.x (columns
-// inside the pivot block are left alone).
-__global__ void __launch_bounds__(256) k_gj_row_panel(double* A, int n, int k0, int kw, const double* P) {
+// inside the pivot block are left alone). In this and the next two kernels
+// blockIdx.z selects the matrix A + z * sa of a batch and its pivot inverse
+// P + z * BK * BK.
+__global__ void __launch_bounds__(256) k_gj_row_panel(double* A, int n, int k0, int kw, const double* P, long long sa) {
+  A += blockIdx.z * sa;
+  P += blockIdx.z * BK * BK;
   __shared__ double sp[BK][BK + 1], sb[BK][TS + 1];
   const int j0 = blockIdx.x * TS, jw = min(TS, n - j0);
   for (int e = threadIdx.x; e < BK * BK; e += blockDim.x) {
@@ -396,7 +409,8 @@ __global__ void __launch_bounds__(256) k_gj_row_panel(double* A, int n, int k0, 
 // A_ij -= A_ik A_kj on tile (blockIdx.y, blockIdx.x), rows and columns
 // inside the pivot block excluded. Thread (tr, tc) owns rows tr + 16a and
 // columns tc + 16b of the tile, a, b < 4.
-__global__ void __launch_bounds__(256) k_gj_trailing(double* A, int n, int k0, int kw) {
+__global__ void __launch_bounds__(256) k_gj_trailing(double* A, int n, int k0, int kw, long long stride) {
+  A += blockIdx.z * stride;
   __shared__ double sa[TS][BK + 1], sb[BK][TS + 1];
   const int i0 = blockIdx.y * TS, iw = min(TS, n - i0), j0 = blockIdx.x * TS, jw = min(TS, n - j0);
   for (int e = threadIdx.x; e < TS * BK; e += blockDim.x) {
@@ -435,7 +449,10 @@ __global__ void __launch_bounds__(256) k_gj_trailing(double* A, int n, int k0, i
 
 // A_ik <- -A_ik P for the rows of row tile blockIdx.x outside the pivot
 // block; the pivot block's rows get A_kk <- P.
-__global__ void __launch_bounds__(256) k_gj_col_panel(double* A, int n, int k0, int kw, const double* P) {
+__global__ void __launch_bounds__(256) k_gj_col_panel(double* A, int n, int k0, int kw, const double* P,
+                                                      long long stride) {
+  A += blockIdx.z * stride;
+  P += blockIdx.z * BK * BK;
   __shared__ double sa[TS][BK + 1], sp[BK][BK + 1];
   const int i0 = blockIdx.x * TS, iw = min(TS, n - i0);
   for (int e = threadIdx.x; e < BK * BK; e += blockDim.x) {
@@ -461,6 +478,8 @@ __global__ void __launch_bounds__(256) k_gj_col_panel(double* A, int n, int k0, 
   }
 }
 
+}  // namespace
+
 struct SolverHandles {
   rocblas_handle h = nullptr;
 };
@@ -477,6 +496,8 @@ int get_handle(hipStream_t s, rocblas_handle* out) {
   *out = g_handles[dev].h;
   return TG_OK;
 }
+
+namespace {
 
 // Constant Galerkin tables per device (FP64, row-major 576 per matrix):
 // Q[t] = P_t^T KE P_t for the 8 children t of a coarse element, and
@@ -646,10 +667,9 @@ extern "C" int tg_assemble_dense(const tg_grid* gp, const double* mats, const in
 
 int dense_inverse_blocked(double* A, int n, hipStream_t st);
 int dense_inverse_cholesky(double* A, int n, hipStream_t st);
-// Above this size the 16-wide pivot steps of the blocked inverse (each a
+// Above kBlockedMaxN the 16-wide pivot steps of the blocked inverse (each a
 // pass over the whole matrix) cost more than rocSOLVER's Cholesky
 // factorisation and inverse (potrf + potri, 45 ms at n = 7,260 on MI355X).
-constexpr int kBlockedMaxN = 2048;
 // Up to this size one workgroup holds the whole matrix in registers (NB x NB
 // FP64 entries per thread, NB = ceil(n / 32)). Each pivot step costs every
 // thread O(NB^2) selects and FMAs whatever the step, so beyond NB = 2 the
@@ -665,8 +685,10 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
     // one workgroup, the matrix in registers: NB x NB entries per thread
     const dim3 g1(1), b1(GJ_THREADS);
     switch ((n + 31) / 32) {
-      case 1: hipLaunchKernelGGL((k_spd_inverse_gj<1>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1); break;
-      default: hipLaunchKernelGGL((k_spd_inverse_gj<2>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1); break;
+      case 1: hipLaunchKernelGGL((k_spd_inverse_gj<1>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1, 0LL, 0LL); break;
+      default:
+        hipLaunchKernelGGL((k_spd_inverse_gj<2>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1, 0LL, 0LL);
+        break;
     }
     TG_LAUNCH_CHECK();
   } else if (n <= kBlockedMaxN) {
@@ -679,37 +701,52 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_HIP_CHECK(hipStreamSynchronize(st));
   if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
   const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
-  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n, 0LL);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
 
-// Blocked Gauss-Jordan with BK-wide pivot blocks; P (the inverted pivot
-// block) lives in the reduction workspace, which no other kernel uses
-// meanwhile on this stream.
-int dense_inverse_blocked(double* A, int n, hipStream_t st) {
+// Blocked Gauss-Jordan with BK-wide pivot blocks on a batch of matrices at
+// stride sa (the same pivot steps on every matrix, one launch per kernel and
+// step); P (the inverted pivot blocks, BK x BK per matrix) lives in the
+// reduction workspace, which no other kernel uses meanwhile on this stream.
+// Leaves each inverse symmetric (mirrored from its upper triangle); a pivot
+// <= 0 sets info (cleared by the caller).
+namespace tg {
+int spd_inverse_batched(double* A, int n, int batch, long long sa, hipStream_t st) {
+  TG_REQUIRE(n >= 1 && n <= kBlockedMaxN && batch >= 1 && (long long)batch * BK * BK <= kMaxPartials,
+             "batched inverse: bad shape");
   int* info_d = workspace().flags;
   double* P = workspace().partials;
-  const int nt = (n + TS - 1) / TS;
+  const unsigned nt = (unsigned)((n + TS - 1) / TS), nb = (unsigned)batch;
   for (int k0 = 0; k0 < n; k0 += BK) {
     const int kw = n - k0 < BK ? n - k0 : BK;
-    hipLaunchKernelGGL((k_spd_inverse_gj<1>), dim3(1), dim3(GJ_THREADS), 0, st, A + (long long)k0 * n + k0, n, kw, P,
-                       BK, info_d, k0, k0 == 0 ? 1 : 0);
+    hipLaunchKernelGGL((k_spd_inverse_gj<1>), dim3(nb), dim3(GJ_THREADS), 0, st, A + (long long)k0 * n + k0, n, kw,
+                       P, BK, info_d, k0, 0, sa, (long long)BK * BK);
     TG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_row_panel, dim3(nt), dim3(256), 0, st, A, n, k0, kw, P);
+    hipLaunchKernelGGL(k_gj_row_panel, dim3(nt, 1, nb), dim3(256), 0, st, A, n, k0, kw, P, sa);
     TG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_trailing, dim3(nt, nt), dim3(256), 0, st, A, n, k0, kw);
+    hipLaunchKernelGGL(k_gj_trailing, dim3(nt, nt, nb), dim3(256), 0, st, A, n, k0, kw, sa);
     TG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_col_panel, dim3(nt), dim3(256), 0, st, A, n, k0, kw, P);
+    hipLaunchKernelGGL(k_gj_col_panel, dim3(nt, 1, nb), dim3(256), 0, st, A, n, k0, kw, P, sa);
     TG_LAUNCH_CHECK();
   }
+  const long long nn = (long long)n * n;   // exact symmetry
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256), nb), dim3(256), 0, st, A, n, sa);
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
+}  // namespace tg
+
+int dense_inverse_blocked(double* A, int n, hipStream_t st) {
+  int* info_d = workspace().flags;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, info_d, 0);
+  TG_LAUNCH_CHECK();
+  if (int r = spd_inverse_batched(A, n, 1, 0, st)) return r;
   int info = 0;
   TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
   TG_HIP_CHECK(hipStreamSynchronize(st));
   if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
-  const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
-  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
-  TG_LAUNCH_CHECK();
   return TG_OK;
 }
 
@@ -736,7 +773,7 @@ int dense_inverse_cholesky(double* A, int n, hipStream_t st) {
   if (rs != rocblas_status_success) return set_error(TG_ERR_HIP, "rocsolver potrf/potri status %d", (int)rs);
   if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (Cholesky info=%d)", (int)info);
   const long long nn = (long long)n * n;
-  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n);
+  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n, 0LL);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

@@ -33,6 +33,8 @@ in FP32.
 
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
@@ -43,6 +45,12 @@ from . import lib as L
 class _Level:
     pass
 
+
+# Coarsest levels with more free DOFs than the dense inverse handles well
+# (its blocked Gauss-Jordan limit) go to block cyclic reduction, whose plane
+# blocks use the same inverse and so share that size limit.
+BCR_MIN_DOFS = 2048
+BCR_MAX_PLANE = 2048
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
@@ -107,7 +115,7 @@ class GMG:
                  power_iters: int = 20, coarse_power_iters: int = 10, seed: int = 7, level1: str = "q8",
                  nrhs: int = 1, coarse_stencil: int = L.TG_OP_STENCIL_F32, coarse_precisions=None,
                  smoother: str = "chebyshev",
-                 omega: float = 0.5, cycle: str = "v"):
+                 omega: float = 0.5, cycle: str = "v", coarse_solver: str = "auto"):
         if n_levels < 2:
             raise ValueError("the GMG preconditioner needs n_levels >= 2")
         if coarse_stencil not in (L.TG_OP_STENCIL_F32, L.TG_OP_STENCIL_BF16):
@@ -133,6 +141,8 @@ class GMG:
             raise ValueError(f"smoother must be 'chebyshev' or 'jacobi'; got {smoother!r}")
         if cycle not in ("v", "w"):
             raise ValueError(f"cycle must be 'v' or 'w'; got {cycle!r}")
+        if coarse_solver not in ("auto", "dense", "bcr"):
+            raise ValueError(f"coarse_solver must be 'auto', 'dense' or 'bcr'; got {coarse_solver!r}")
         self.smoother, self.omega, self.cycle = smoother, float(omega), cycle
         self.power_iters = power_iters
         self.coarse_power_iters = coarse_power_iters
@@ -214,11 +224,29 @@ class GMG:
         self.n_coarse = int(free.size)
         if self.n_coarse == 0:
             raise ValueError("coarsest level has no free DOFs")
-        fidx = -np.ones(last.n, dtype=np.int32)
-        fidx[free] = np.arange(free.size, dtype=np.int32)
-        self.coarse_fidx = torch.from_numpy(fidx).to(device)
-        self.coarse_list = torch.from_numpy(free.astype(np.int32)).to(device)
-        self.coarse_A = torch.zeros(self.n_coarse * self.n_coarse, dtype=torch.float64, device=device)
+        # Coarsest solve (multigrid_v4.py:781-801, 977-978): the explicit dense
+        # inverse, one GEMV per cycle, up to BCR_MIN_DOFS free DOFs; above,
+        # block cyclic reduction over the level's x-planes (an exact block
+        # Cholesky whose setup is batched plane-block work instead of an n^3
+        # dense factorisation and inverse), when a plane fits a block.
+        plane = 3 * (last.dims[1] + 1) * (last.dims[2] + 1)
+        bcr_ok = plane <= BCR_MAX_PLANE and last.dims[0] >= 1
+        if coarse_solver == "bcr" and not bcr_ok:
+            raise ValueError(f"coarse_solver='bcr' needs x-planes of at most {BCR_MAX_PLANE} DOFs; got {plane}")
+        self.coarse_bcr = coarse_solver == "bcr" or (coarse_solver == "auto" and bcr_ok
+                                                     and self.n_coarse > BCR_MIN_DOFS)
+        self.coarse_A = self.bcr_work = self.bcr_vec = None
+        if self.coarse_bcr:
+            nw = C.c_int64(0)
+            L.call("tg_bcr_size", last.grid, C.byref(nw))
+            self.bcr_work = torch.zeros(int(nw.value), dtype=torch.float64, device=device)
+            self.bcr_vec = torch.zeros(2 * self.nrhs * last.n, dtype=torch.float64, device=device)
+        else:
+            fidx = -np.ones(last.n, dtype=np.int32)
+            fidx[free] = np.arange(free.size, dtype=np.int32)
+            self.coarse_fidx = torch.from_numpy(fidx).to(device)
+            self.coarse_list = torch.from_numpy(free.astype(np.int32)).to(device)
+            self.coarse_A = torch.zeros(self.n_coarse * self.n_coarse, dtype=torch.float64, device=device)
         self.scal = torch.zeros(8, dtype=torch.float64, device=device)
         self._emax_at_estimate = float("nan")
         self.is_setup = False
@@ -278,9 +306,12 @@ class GMG:
                            L.ptr(c.mats64), L.ptr(c.stencil), stream)
                     L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
         last = self.levels[-1]
-        L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
-               L.ptr(self.coarse_A), stream)
-        L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
+        if self.coarse_bcr:
+            L.call("tg_bcr_setup", last.grid, L.ptr(last.mask), L.ptr(last.mats64), L.ptr(self.bcr_work), stream)
+        else:
+            L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
+                   L.ptr(self.coarse_A), stream)
+            L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
         if reestimate:
             for lv in range(last_lv):
                 iters = self.power_iters if lv == 0 else self.coarse_power_iters
@@ -420,8 +451,12 @@ class GMG:
     def _vcycle(self, lv: int, rhs, stream):
         Lv = self.levels[lv]
         if lv == len(self.levels) - 1:
-            L.call("tg_coarsest_apply_b", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
-                   self.nrhs, L.ptr(rhs), L.ptr(Lv.x), stream)
+            if self.coarse_bcr:
+                L.call("tg_bcr_apply_b", Lv.grid, L.ptr(self.bcr_work), self.nrhs, L.ptr(rhs), L.ptr(Lv.x),
+                       L.ptr(self.bcr_vec), stream)
+            else:
+                L.call("tg_coarsest_apply_b", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self.coarse_list), Lv.n,
+                       self.nrhs, L.ptr(rhs), L.ptr(Lv.x), stream)
             return Lv.x
         self._vcycle_down(lv, rhs, stream)
         return self._vcycle_up(lv, stream)
@@ -432,7 +467,8 @@ class GMG:
         the modulus buffers and the setup-time buffers."""
         return (tuple(float(Lv.lam) for Lv in self.levels), self.fine_degree, tuple(self.coarse_degrees),
                 self.safety, self.frac, L.ptr(self.E32), L.ptr(self.E64),
-                tuple(L.ptr(Lv.stencil) for Lv in self.levels), L.ptr(self.coarse_A), self.nrhs, self.smoother,
+                tuple(L.ptr(Lv.stencil) for Lv in self.levels), L.ptr(self.coarse_A), L.ptr(self.bcr_work),
+                L.ptr(self.bcr_vec), self.nrhs, self.smoother,
                 self.omega, self.cycle)
 
     def apply(self, r, stream):

@@ -70,6 +70,9 @@ _SIGS = {
     "tg_assemble_stencil_bf16": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_dense_spd_inverse": [_vp, _i32, _vp],
     "tg_coarsest_apply": [_vp, _i32, _vp, _i64, _vp, _vp, _vp],
+    "tg_bcr_size": [C.POINTER(TgGrid), _vp],
+    "tg_bcr_setup": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
+    "tg_bcr_apply_b": [C.POINTER(TgGrid), _vp, _i32, _vp, _vp, _vp, _vp],
     "tg_dot32": [_i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "tg_dot64": [_i64, _vp, _vp, _vp, _vp],
     "tg_pcg_xr": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
