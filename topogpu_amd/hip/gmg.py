@@ -46,10 +46,13 @@ class _Level:
     pass
 
 
-# Coarsest levels with more free DOFs than the dense inverse handles well
-# (its blocked Gauss-Jordan limit) go to block cyclic reduction, whose plane
-# blocks use the same inverse and so share that size limit.
-BCR_MIN_DOFS = 2048
+# Coarsest levels above BCR_MIN_DOFS free DOFs go to block cyclic reduction:
+# on 160x80x80, 5 levels (1,080 coarsest DOFs) run at 9.60 ms per SIMP
+# iteration with it against 10.29 ms with the dense inverse (whose setup is
+# ~70 pivot-step launches); 6 levels (135 DOFs) favour the dense inverse,
+# 9.62 against 9.78 ms. A plane block goes through the blocked Gauss-Jordan
+# inverse, hence its size limit.
+BCR_MIN_DOFS = 1024
 BCR_MAX_PLANE = 2048
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",

@@ -85,7 +85,8 @@ class HipStateSolver:
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = True, speculate: bool = True,
                  smoother: str = "chebyshev", omega: float = 0.5, cycle: str = "v", outer_solver: str = "pcg",
-                 restart: int = 32, probe_quality: bool = False, level_precisions=None, device=None):
+                 restart: int = 32, probe_quality: bool = False, level_precisions=None, coarse_solver: str = "auto",
+                 device=None):
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a ROCm GPU (torch.cuda.is_available() is False)")
         if level_precisions is not None:
@@ -139,7 +140,7 @@ class HipStateSolver:
                        coarse_degree=coarse_degree, cheb_lower_frac=cheb_lower_frac, cheb_safety=cheb_safety,
                        level1=level1, nrhs=self.nrhs, coarse_stencil=ops[5], smoother=smoother, omega=omega,
                        cycle=cycle, coarse_precisions=None if level_precisions is None else level_precisions[1:],
-                       device=self.dev)
+                       coarse_solver=coarse_solver, device=self.dev)
         nb = self.nrhs * self.n
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.u = torch.zeros(nb, dtype=torch.float64, device=self.dev)
