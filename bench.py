@@ -170,24 +170,42 @@ def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated
 def saturated_kernel_times(counter, steps: int, reps: int = 50) -> dict:
     """Device duration of every accounted (level, kernel, epilogue) launch
     of the timed region: its last launch (live operands of the last SIMP
-    iteration) issued again reps times back to back between two HIP events
-    on the kernels' stream, so host launch gaps do not enter. Run after the
-    timed region; the replays overwrite the solver's work vectors."""
+    iteration) captured reps times back to back into one HIP graph on a side
+    stream and replayed between two HIP events recorded on that stream, so
+    host launch rates and gaps do not enter (a kernel shorter than the host's
+    launch interval would otherwise be timed at that interval). The average
+    is the kernel's duration plus the graph's dispatch gap, about 1 us. A
+    reducing launch includes its one-block finalize. Run after the timed
+    region; the replays overwrite the solver's work vectors."""
+    import ctypes
+
     import torch
 
+    from topogpu_amd.hip import lib as L
     from topogpu_amd.hip.gmg import EP_NAMES
 
     out = {}
+    side = torch.cuda.Stream()
+    st = side.cuda_stream
     for (name, mode), (fn, nb) in counter.replay.items():
-        for _ in range(3):
-            fn()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
-        for _ in range(reps):
-            fn()
-        e.record()
         torch.cuda.synchronize()
-        us = 1e3 * s.elapsed_time(e) / reps
+        L.call("tg_graph_begin", st)
+        h = ctypes.c_void_p()
+        try:
+            for _ in range(reps):
+                fn(st)
+        finally:
+            L.load().tg_graph_end(st, ctypes.byref(h))
+        try:
+            L.call("tg_graph_launch", h.value, st)   # warm-up replay
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(side)
+            L.call("tg_graph_launch", h.value, st)
+            e.record(side)
+            torch.cuda.synchronize()
+            us = 1e3 * s.elapsed_time(e) / reps
+        finally:
+            L.call("tg_graph_destroy", h.value)
         per_step = counter.counts[(name, mode)] / steps
         out[f"{name}/{EP_NAMES[mode]}"] = {
             "avg_us": us, "bytes_per_launch": nb, "gbps": nb / (us * 1e-6) / 1e9,

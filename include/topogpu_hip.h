@@ -243,7 +243,8 @@ int tg_assemble_dense(const tg_grid* g, const double* mats64, const int32_t* fre
 int tg_dense_spd_inverse(double* A, int32_t n, void* stream);
 
 /* ec = scatter(Ainv * gather(rc)) over the free DOFs of the coarsest level;
- * fixed DOFs of ec are written as 0. free_list[i] = full DOF of compact i.
+ * fixed DOFs of ec are written as 0. free_list[i] = full DOF of compact i,
+ * in ascending order.
  * Replaces _solve_coarsest (multigrid_v4.py:969-984). */
 int tg_coarsest_apply(const double* Ainv, int32_t n_free, const int32_t* free_list,
                       int64_t n_full, const float* rc, float* ec, void* stream);

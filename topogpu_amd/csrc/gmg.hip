@@ -261,11 +261,27 @@ __global__ void k_symmetrize_from_upper(double* A, int n, long long sa) {
 }
 
 // One wavefront per row: ec[free[i]] = sum_j Ainv[i][j] rc[free[j]]; RHS
-// blockIdx.y of a batch uses the slices at blockIdx.y * n_full.
+// blockIdx.y of a batch uses the slices at blockIdx.y * n_full. Blocks past
+// the n / 4 row blocks write the fixed entries of ec as zero, one thread per
+// entry (an entry is fixed when a binary search of the ascending free list
+// misses it), so no entry is written twice and no fill launch is needed.
 __global__ void k_coarsest_apply(const double* __restrict__ Ainv, int n, const int* __restrict__ fl,
                                  long long n_full, const float* __restrict__ rc, float* __restrict__ ec) {
   rc += blockIdx.y * n_full;
   ec += blockIdx.y * n_full;
+  const int row_blocks = (n + 3) / 4;
+  if ((int)blockIdx.x >= row_blocks) {
+    const long long e = (long long)(blockIdx.x - row_blocks) * blockDim.x + threadIdx.x;
+    if (e >= n_full) return;
+    int lo = 0, hi = n;   // first free index >= e
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (fl[mid] < e) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo == n || fl[lo] != e) ec[e] = 0.0f;
+    return;
+  }
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
@@ -782,9 +798,10 @@ extern "C" int tg_coarsest_apply_b(const double* Ainv, int32_t n, const int32_t*
                                    const float* rc, float* ec, void* s) {
   TG_REQUIRE(Ainv && fl && rc && ec && n > 0 && n_full >= n && nrhs >= 1 && nrhs <= 64,
              "coarsest_apply: bad argument");
-  if (int r = tg_fill32(n_full * nrhs, ec, 0.0f, s)) return r;
-  hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)((n + 3) / 4), nrhs), dim3(256), 0, (hipStream_t)s, Ainv, n,
-                     fl, (long long)n_full, rc, ec);
+  const long long blocks = (n + 3) / 4 + (n_full + 255) / 256;
+  TG_REQUIRE(blocks < (1LL << 31), "coarsest_apply: too large");
+  hipLaunchKernelGGL(k_coarsest_apply, dim3((unsigned)blocks, nrhs), dim3(256), 0, (hipStream_t)s, Ainv, n, fl,
+                     (long long)n_full, rc, ec);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }

@@ -98,16 +98,17 @@ def launch_op(timer, lv: int, op, u_ptr, y_ptr, ep, stream, n_dof: int, n_elem: 
         L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream)
         return
     timer.launch(kernel_key(lv, op.kind), nrhs * op_bytes(op.kind, n_dof, n_elem, ep.mode),
-                 lambda: L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, stream), ep.mode)
+                 lambda st=stream: L.call("tg_apply32_b", op, nrhs, u_ptr, y_ptr, ep, st), ep.mode)
 
 
 def launch_transfer(timer, name: str, lv: int, nbytes: float, *args) -> None:
     """tg_restrict32_b / tg_prolong32_b between levels lv and lv + 1,
-    accounted when a timer is attached."""
+    accounted when a timer is attached (args end with the stream)."""
     if timer is None:
         L.call(name, *args)
         return
-    timer.launch(f"L{lv} {name[3:-2]}", nbytes, lambda: L.call(name, *args))
+    head, stream = args[:-1], args[-1]
+    timer.launch(f"L{lv} {name[3:-2]}", nbytes, lambda st=stream: L.call(name, *head, st))
 
 
 class GMG:

@@ -6,8 +6,9 @@ libtopohip call here receives), so the measured durations are those of the
 kernels themselves. Events cost host time per launch, so a timed benchmark
 region uses ``events=False``: launches and algorithmic bytes are counted per
 (kernel, epilogue mode) and nothing else is done. The last launch of each
-key is kept as a closure (``replay``), so that its duration can be measured
-afterwards by launching it again back to back.
+key is kept as a closure (``replay``; called with no argument it launches on
+its original stream, called with a stream handle on that stream), so that its
+duration can be measured afterwards by launching it again back to back.
 """
 
 from __future__ import annotations
