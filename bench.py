@@ -61,27 +61,34 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# HBM bytes per launch of a level-0 (kernel, epilogue) pair at 160x80x80,
-# measured with hardware counters (FETCH_SIZE and WRITE_SIZE runs of
-# ``bench.py --kernel-probe``, scaled by the counts of a 256 MiB copy); the
-# summary file holds the commands and the figures.
-TRAFFIC_PROFILES = {"L0 hex_bf16_mfma/CHEB_R": "profiles/hex_mfma_cheb_r_160x80x80.md",
-                    "L0 hex_bf16x3_mfma/DOT": "profiles/hex_mfma_x3_dot_160x80x80.md",
-                    "L0 hex_bf16_mfma/CHEB_FIRST": "profiles/hex_mfma_cheb_first_160x80x80.md"}
+# HBM bytes and MFMA busy fraction per launch of the probed (level, kernel,
+# epilogue) launches at 160x80x80, measured with hardware counters
+# (FETCH_SIZE / WRITE_SIZE runs of ``bench.py --kernel-probe`` scaled by the
+# counts of a 256 MiB copy, and SQ_VALU_MFMA_BUSY_CYCLES); the summary file
+# holds the commands and the figures, one "<field>[<probe name>]: <value>"
+# line per kernel.
+COUNTER_PROFILE = "profiles/counters_kernel_probe_160x80x80.md"
+PROBE_NAMES = {"L0 hex_bf16_mfma/CHEB_R": "L0 CHEB_R", "L0 hex_bf16_mfma/CHEB_FIRST": "L0 CHEB_FIRST",
+               "L0 hex_bf16_mfma/RESID_UPDATE": "L0 RESID_UPDATE", "L0 hex_bf16_mfma/CHEB_LAST": "L0 CHEB_LAST",
+               "L0 hex_bf16x3_mfma/DOT": "L0 split DOT", "L0 hex_bf16_mfma/STORE": "L0 STORE",
+               "L1 galerkin_q8_bf16_mfma/RESID_FIRST": "L1 RESID_FIRST",
+               "L1 galerkin_q8_bf16_mfma/CHEB_LAST": "L1 CHEB_LAST", "L2 stencil_bf16/CHEB_R": "L2 CHEB_R",
+               "L3 stencil_bf16/CHEB_R": "L3 CHEB_R"}
 
 
 def profiled_value(kernel: str, field: str):
-    """A figure ("<field>: <value>" line) from the kernel's counter summary
-    under profiles/, or (None, None)."""
-    path = TRAFFIC_PROFILES.get(kernel)
-    if path is None:
+    """A counter figure for the kernel from the counter summary under
+    profiles/, as (value, path), or (None, None) when it holds none."""
+    probe = PROBE_NAMES.get(kernel)
+    if probe is None:
         return None, None
-    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), COUNTER_PROFILE)
+    tag = f"{field}[{probe}]:"
     try:
         with open(full) as f:
             for line in f:
-                if line.startswith(field + ":"):
-                    return float(line.split(":")[1]), path
+                if line.startswith(tag):
+                    return float(line[len(tag):]), COUNTER_PROFILE
     except OSError:
         pass
     return None, None
