@@ -80,7 +80,7 @@ class HipStateSolver:
     """
 
     def __init__(self, dims, free: np.ndarray, F: np.ndarray, *, precision: str = "bf16", n_levels="auto",
-                 fine_degree: int = 1, coarse_degree=(1, 2, 8), cheb_lower_frac: float = 1.0 / 30.0,
+                 fine_degree: int = 1, coarse_degree=(1, 2, 10, 4), cheb_lower_frac: float = 1.0 / 30.0,
                  cheb_safety: float = 1.1,
                  tol: float = 1e-5, max_krylov: int = 800, inner_rtol: float = 1e-2, max_restarts: int = 50,
                  warm_start: bool = True, level1: str = "q8", graphs: bool = True, speculate: bool = True,
