@@ -518,8 +518,9 @@ def test_not_spd_is_reported():
     A = t(np.array([1.0, 2.0, 2.0, 1.0]))
     with pytest.raises(L.TopoHipError, match="not SPD"):
         L.call("tg_dense_spd_inverse", L.ptr(A), 2, stream())
-    # one-workgroup path, then the blocked path in its last, middle and first block
-    for n, bad in ((50, 33), (400, 399), (200, 100), (130, 0)):
+    # the register path, the LDS path (in its first, a middle and its last
+    # pivot block), then the blocked path in its last, middle and first block
+    for n, bad in ((50, 33), (130, 0), (135, 70), (140, 139), (400, 399), (200, 100), (170, 0)):
         B = np.eye(n)
         B[bad, bad] = -1.0
         with pytest.raises(L.TopoHipError, match=f"not SPD \\(pivot {bad + 1}\\)"):
@@ -530,11 +531,12 @@ def test_not_spd_is_reported():
         L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), 2100, stream())
 
 
-@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 97, 135, 160, 161, 192, 193, 600, 1080, 2100])
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 97, 135, 140, 141, 160, 161, 192, 193, 600, 1080, 2100])
 def test_dense_spd_inverse_sizes(n):
-    """One-workgroup Gauss-Jordan (n <= 64), blocked Gauss-Jordan (16-wide
-    pivot blocks, n <= 2048) and the library Cholesky inverse (above) invert
-    an ill-conditioned SPD matrix to FP64 accuracy, symmetric."""
+    """One-workgroup Gauss-Jordan in registers (n <= 64) and in LDS (n <=
+    140), blocked Gauss-Jordan (16-wide pivot blocks, n <= 2048) and the
+    library Cholesky inverse (above) invert an ill-conditioned SPD matrix to
+    FP64 accuracy, symmetric."""
     rng = np.random.default_rng(n)
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     ev = np.logspace(0, 8, n)   # condition number 1e8

@@ -494,6 +494,131 @@ __global__ void __launch_bounds__(256) k_gj_col_panel(double* A, int n, int k0, 
   }
 }
 
+// The blocked Gauss-Jordan above in one workgroup with the whole matrix in
+// LDS (n <= kLdsInvMaxN): each BK-wide pivot block is inverted in registers
+// (threads (r, c) < BK, the scalar steps of k_spd_inverse_gj), then the row
+// panel, the trailing update and the column panel are formed from LDS with
+// the same sums in the same order as k_gj_row_panel, k_gj_trailing and
+// k_gj_col_panel. One launch replaces four per pivot block; the upper
+// triangle is mirrored before the write-back, as k_symmetrize_from_upper
+// does.
+constexpr int kLdsInvMaxN = 140;   // 140^2 doubles = 153 KiB of LDS
+constexpr int kLdsInvNB = (kLdsInvMaxN + 31) / 32;
+
+__global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_lds(double* __restrict__ A, int n, int* info) {
+  __shared__ double sA[kLdsInvMaxN * kLdsInvMaxN];
+  __shared__ double sP[BK * BK];
+  __shared__ double prow[BK], pcol[BK];
+  const int tid = threadIdx.x;
+  const int nn = n * n;
+  for (int e = tid; e < nn; e += GJ_THREADS) sA[e] = A[e];
+  if (tid == 0) *info = 0;
+  __syncthreads();
+  const int pr_r = tid >> 4, pr_c = tid & 15;    // pivot-block entry of threads < BK * BK
+  const int ti = tid >> 5, tj = tid & 31;          // trailing-update layout
+  for (int k0 = 0; k0 < n; k0 += BK) {
+    const int kw = n - k0 < BK ? n - k0 : BK;
+    // P = inverse of the pivot block (scalar Gauss-Jordan, as k_spd_inverse_gj)
+    const bool pin = tid < BK * BK && pr_r < kw && pr_c < kw;
+    double a = pin ? sA[(k0 + pr_r) * n + k0 + pr_c] : 0.0;
+    for (int k = 0; k < kw; ++k) {
+      if (pin && pr_c == k) pcol[pr_r] = a;
+      if (pin && pr_r == k) prow[pr_c] = a;
+      __syncthreads();
+      const double p = pcol[k];   // uniform
+      if (!(p > 0.0)) {
+        if (tid == 0) *info = k0 + k + 1;
+        return;
+      }
+      const double ip = 1.0 / p;
+      if (pin) {
+        const double prj = (pr_c == k ? 1.0 : prow[pr_c]) * ip;
+        const double pci = pcol[pr_r];
+        a = (pr_r == k) ? prj : (pr_c == k ? 0.0 : a) - pci * prj;
+      }
+      __syncthreads();
+    }
+    if (pin) sP[pr_r * BK + pr_c] = a;
+    __syncthreads();
+    // row panel: A_kj <- P A_kj (k_gj_row_panel's sums)
+    constexpr int RPW = (BK * kLdsInvMaxN + GJ_THREADS - 1) / GJ_THREADS;
+    double rp[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int e = tid + q * GJ_THREADS, r = e / n, j = e - r * n;
+      double acc = 0.0;
+      if (r < kw)
+        for (int kk = 0; kk < kw; ++kk) acc = fma(sP[r * BK + kk], sA[(k0 + kk) * n + j], acc);
+      rp[q] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int e = tid + q * GJ_THREADS, r = e / n, j = e - r * n;
+      if (r < kw && (j < k0 || j >= k0 + kw)) sA[(k0 + r) * n + j] = rp[q];
+    }
+    __syncthreads();
+    // trailing update: A_ij -= A_ik A_kj (k_gj_trailing's sums)
+    double acc[kLdsInvNB][kLdsInvNB] = {};
+    for (int kk = 0; kk < kw; ++kk) {
+      double l[kLdsInvNB], r[kLdsInvNB];
+#pragma unroll
+      for (int x = 0; x < kLdsInvNB; ++x) {
+        const int i = ti + 32 * x;
+        l[x] = i < n ? sA[i * n + k0 + kk] : 0.0;
+      }
+#pragma unroll
+      for (int y = 0; y < kLdsInvNB; ++y) {
+        const int j = tj + 32 * y;
+        r[y] = j < n ? sA[(k0 + kk) * n + j] : 0.0;
+      }
+#pragma unroll
+      for (int x = 0; x < kLdsInvNB; ++x)
+#pragma unroll
+        for (int y = 0; y < kLdsInvNB; ++y) acc[x][y] = fma(l[x], r[y], acc[x][y]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < kLdsInvNB; ++x) {
+      const int i = ti + 32 * x;
+#pragma unroll
+      for (int y = 0; y < kLdsInvNB; ++y) {
+        const int j = tj + 32 * y;
+        if (i < n && j < n && (i < k0 || i >= k0 + kw) && (j < k0 || j >= k0 + kw)) sA[i * n + j] -= acc[x][y];
+      }
+    }
+    // column panel from the old A_ik, which the trailing update leaves alone
+    // (k_gj_col_panel's sums); written once every thread has formed its own
+    constexpr int CPW = (kLdsInvMaxN * BK + GJ_THREADS - 1) / GJ_THREADS;
+    double cp[CPW];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int e = tid + q * GJ_THREADS, i = e / BK, c = e - i * BK;
+      double v = 0.0;
+      if (i < n && c < kw) {
+        if (i >= k0 && i < k0 + kw) {
+          v = sP[(i - k0) * BK + c];
+        } else {
+          for (int kk = 0; kk < kw; ++kk) v = fma(sA[i * n + k0 + kk], sP[kk * BK + c], v);
+          v = -v;
+        }
+      }
+      cp[q] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int e = tid + q * GJ_THREADS, i = e / BK, c = e - i * BK;
+      if (i < n && c < kw) sA[i * n + k0 + c] = cp[q];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nn; e += GJ_THREADS) {
+    const int r = e / n, c = e - r * n;
+    A[e] = r > c ? sA[c * n + r] : sA[e];
+  }
+}
+
 }  // namespace
 
 struct SolverHandles {
@@ -707,6 +832,15 @@ extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
         break;
     }
     TG_LAUNCH_CHECK();
+  } else if (n <= kLdsInvMaxN) {
+    // one workgroup, the matrix in LDS (mirrored from its upper triangle)
+    hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(GJ_THREADS), 0, st, A, n, info_d);
+    TG_LAUNCH_CHECK();
+    int info = 0;
+    TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
+    TG_HIP_CHECK(hipStreamSynchronize(st));
+    if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
+    return TG_OK;
   } else if (n <= kBlockedMaxN) {
     return dense_inverse_blocked(A, n, st);
   } else {
