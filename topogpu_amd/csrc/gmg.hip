@@ -511,6 +511,8 @@ __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_lds(double* __restri
   __shared__ double prow[BK], pcol[BK];
   const int tid = threadIdx.x;
   const int nn = n * n;
+  // several loads of the matrix in flight per thread (an unrolled loop)
+#pragma unroll 6
   for (int e = tid; e < nn; e += GJ_THREADS) sA[e] = A[e];
   if (tid == 0) *info = 0;
   __syncthreads();
@@ -613,6 +615,7 @@ __global__ void __launch_bounds__(GJ_THREADS) k_spd_inverse_lds(double* __restri
     }
     __syncthreads();
   }
+#pragma unroll 6
   for (int e = tid; e < nn; e += GJ_THREADS) {
     const int r = e / n, c = e - r * n;
     A[e] = r > c ? sA[c * n + r] : sA[e];
