@@ -125,7 +125,8 @@ def test_hex_mfma_grids_larger_than_one_wave_of_workgroups(dims):
     rounds of workgroups, tiles cut at every grid face): every tile is
     computed once, against the FP64 device operator (itself checked against
     the oracle above), for STORE, DOT and CHEB_R, one and two right-hand
-    sides."""
+    sides. BF16 single pass to 4e-3 relative (the 128x64x64 matvec of the
+    same input distributions measures 1.8e-3), split BF16 to 1e-5."""
     rng = np.random.default_rng(31)
     nn = int(np.prod([d + 1 for d in dims]))
     n = 3 * nn
@@ -142,7 +143,7 @@ def test_hex_mfma_grids_larger_than_one_wave_of_workgroups(dims):
                None, L.ptr(y64[k * n:(k + 1) * n]), None, None, stream())
     ref = h(y64)
     assert np.abs(ref).max() > 0
-    for kind, tol in ((L.TG_OP_HEX_BF16X3, 1e-5), (L.TG_OP_HEX_BF16, 1e-2)):
+    for kind, tol in ((L.TG_OP_HEX_BF16X3, 1e-5), (L.TG_OP_HEX_BF16, 4e-3)):
         op = L.TgOp32(kind, L.grid(dims), L.ptr(dm), L.ptr(dE32), None)
         for nr in (1, 2):
             y = torch.full((nr * n,), np.nan, dtype=torch.float32, device=DEV)
