@@ -234,13 +234,23 @@ int tg_assemble_dense(const tg_grid* g, const double* mats64, const int32_t* fre
 
 /* In-place explicit inverse A^-1 (full symmetric) of a dense FP64 SPD
  * matrix A (n x n, row-major): Gauss-Jordan elimination without pivoting in
- * one workgroup (n <= 64) or in 16-wide pivot blocks with tiled updates
- * (n <= 2048), rocSOLVER Cholesky factorisation and inverse above.
+ * one workgroup with the matrix in registers (n <= 64), in 16-wide pivot
+ * blocks in one workgroup with the matrix in LDS (n <= 140), in 16-wide
+ * pivot blocks with tiled updates over the whole GPU (n <= 2048), rocSOLVER
+ * Cholesky factorisation and inverse above.
  * Host-synchronous. Replaces cp.linalg.cholesky + solve_triangular
  * (multigrid_v4.py:781-801, 977-978): the V-cycle then applies the coarsest
  * solve as one matrix-vector product. Returns TG_ERR_NOT_SPD when a pivot is
  * not positive. */
 int tg_dense_spd_inverse(double* A, int32_t n, void* stream);
+
+/* tg_dense_spd_inverse for n <= 140 (the one-workgroup inverses), queued on
+ * the stream without a host wait: *info (device memory) is set to 0, or to
+ * the first non-positive pivot + 1, when the kernels run, and A then holds
+ * A^-1 only if *info is 0. Lets the caller overlap the inverse, which
+ * occupies one compute unit, with other work and read *info later. Same
+ * reference lines as tg_dense_spd_inverse (multigrid_v4.py:781-801). */
+int tg_dense_spd_inverse_async(double* A, int32_t n, int32_t* info, void* stream);
 
 /* ec = scatter(Ainv * gather(rc)) over the free DOFs of the coarsest level;
  * fixed DOFs of ec are written as 0. free_list[i] = full DOF of compact i,

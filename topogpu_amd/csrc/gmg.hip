@@ -814,49 +814,55 @@ int dense_inverse_cholesky(double* A, int n, hipStream_t st);
 // Above kBlockedMaxN the 16-wide pivot steps of the blocked inverse (each a
 // pass over the whole matrix) cost more than rocSOLVER's Cholesky
 // factorisation and inverse (potrf + potri, 45 ms at n = 7,260 on MI355X).
-// Up to this size one workgroup holds the whole matrix in registers (NB x NB
-// FP64 entries per thread, NB = ceil(n / 32)). Each pivot step costs every
+// Up to kRegisterMaxN one workgroup holds the whole matrix in registers (NB x
+// NB FP64 entries per thread, NB = ceil(n / 32)). Each pivot step costs every
 // thread O(NB^2) selects and FMAs whatever the step, so beyond NB = 2 the
-// blocked form is faster: at n = 135, 286 us for NB = 5 against 255 us.
+// blocked form is faster: at n = 135, 286 us for NB = 5 against 255 us for
+// the blocked form in four launches per pivot block, and 160 us for the
+// same blocked steps in one workgroup with the matrix in LDS
+// (k_spd_inverse_lds, up to kLdsInvMaxN).
 constexpr int kRegisterMaxN = 64;
+
+// The one-workgroup inverses (n <= kLdsInvMaxN), queued on st without a host
+// wait: A inverted in place and left symmetric, *info = 0 or the first
+// failing pivot + 1 (each kernel clears it first).
+int queue_one_workgroup_inverse(double* A, int n, int* info, hipStream_t st) {
+  const dim3 g1(1), b1(GJ_THREADS);
+  if (n <= kRegisterMaxN) {
+    // the matrix in registers: NB x NB entries per thread
+    if (n <= 32)
+      hipLaunchKernelGGL((k_spd_inverse_gj<1>), g1, b1, 0, st, A, n, n, A, n, info, 0, 1, 0LL, 0LL);
+    else
+      hipLaunchKernelGGL((k_spd_inverse_gj<2>), g1, b1, 0, st, A, n, n, A, n, info, 0, 1, 0LL, 0LL);
+    TG_LAUNCH_CHECK();
+    const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
+    hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n, 0LL);
+  } else {
+    // the matrix in LDS, mirrored from its upper triangle on the way out
+    hipLaunchKernelGGL(k_spd_inverse_lds, g1, b1, 0, st, A, n, info);
+  }
+  TG_LAUNCH_CHECK();
+  return TG_OK;
+}
 
 extern "C" int tg_dense_spd_inverse(double* A, int32_t n, void* s) {
   TG_REQUIRE(A && n > 0, "dense_spd_inverse: bad argument");
   TG_REQUIRE(::tg::ke_ready(), "tg_init has not been called on this device");
   const hipStream_t st = (hipStream_t)s;
+  if (n > kLdsInvMaxN) return n <= kBlockedMaxN ? dense_inverse_blocked(A, n, st) : dense_inverse_cholesky(A, n, st);
   int* info_d = workspace().flags;
-  if (n <= kRegisterMaxN) {
-    // one workgroup, the matrix in registers: NB x NB entries per thread
-    const dim3 g1(1), b1(GJ_THREADS);
-    switch ((n + 31) / 32) {
-      case 1: hipLaunchKernelGGL((k_spd_inverse_gj<1>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1, 0LL, 0LL); break;
-      default:
-        hipLaunchKernelGGL((k_spd_inverse_gj<2>), g1, b1, 0, st, A, n, n, A, n, info_d, 0, 1, 0LL, 0LL);
-        break;
-    }
-    TG_LAUNCH_CHECK();
-  } else if (n <= kLdsInvMaxN) {
-    // one workgroup, the matrix in LDS (mirrored from its upper triangle)
-    hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(GJ_THREADS), 0, st, A, n, info_d);
-    TG_LAUNCH_CHECK();
-    int info = 0;
-    TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
-    TG_HIP_CHECK(hipStreamSynchronize(st));
-    if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
-    return TG_OK;
-  } else if (n <= kBlockedMaxN) {
-    return dense_inverse_blocked(A, n, st);
-  } else {
-    return dense_inverse_cholesky(A, n, st);
-  }
+  if (int r = queue_one_workgroup_inverse(A, n, info_d, st)) return r;
   int info = 0;
   TG_HIP_CHECK(hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, st));
   TG_HIP_CHECK(hipStreamSynchronize(st));
   if (info != 0) return set_error(TG_ERR_NOT_SPD, "dense inverse: matrix not SPD (pivot %d)", info);
-  const long long nn = (long long)n * n;   // exact symmetry for the V-cycle
-  hipLaunchKernelGGL(k_symmetrize_from_upper, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, n, 0LL);
-  TG_LAUNCH_CHECK();
   return TG_OK;
+}
+
+extern "C" int tg_dense_spd_inverse_async(double* A, int32_t n, int32_t* info, void* s) {
+  TG_REQUIRE(A && info && n > 0, "dense_spd_inverse_async: bad argument");
+  TG_REQUIRE(n <= kLdsInvMaxN, "dense_spd_inverse_async: n = %d above %d", n, kLdsInvMaxN);
+  return queue_one_workgroup_inverse(A, n, info, (hipStream_t)s);
 }
 
 // Blocked Gauss-Jordan with BK-wide pivot blocks on a batch of matrices at

@@ -40,6 +40,7 @@ import torch
 
 from .. import fem
 from . import lib as L
+from .readback import Readback
 
 
 class _Level:
@@ -54,6 +55,8 @@ class _Level:
 # inverse, hence its size limit.
 BCR_MIN_DOFS = 1024
 BCR_MAX_PLANE = 2048
+# largest coarsest level inverted by tg_dense_spd_inverse_async (one workgroup)
+ASYNC_INVERSE_MAX = 140
 
 KERNEL_NAMES = {L.TG_OP_HEX_F32: "hex_f32", L.TG_OP_HEX_BF16: "hex_bf16_mfma",
                 L.TG_OP_HEX_BF16X3: "hex_bf16x3_mfma", L.TG_OP_Q8_F32: "galerkin_q8_f32",
@@ -252,6 +255,20 @@ class GMG:
             self.coarse_list = torch.from_numpy(free.astype(np.int32)).to(device)
             self.coarse_A = torch.zeros(self.n_coarse * self.n_coarse, dtype=torch.float64, device=device)
         self.scal = torch.zeros(8, dtype=torch.float64, device=device)
+        self._rb = Readback(2)
+        # The one-workgroup coarsest inverse occupies one compute unit for
+        # about 160 us at 135 DOFs: it runs on a side stream, beside the
+        # levels' stencil assembly and the start of the solve, and the first
+        # use of the coarsest level waits for it (join); its status reaches
+        # the host with the next read (check).
+        self.async_inverse = not self.coarse_bcr and self.n_coarse <= ASYNC_INVERSE_MAX
+        if self.async_inverse:
+            self._side = torch.cuda.Stream(device)
+            self._ev_dense = torch.cuda.Event()
+            self._ev_inv = torch.cuda.Event()
+            self._inv_info = torch.zeros(1, dtype=torch.int32, device=device)
+            self._rb_info = Readback(1, dtype=torch.int32)
+        self._join_pending = self._check_pending = False
         self._emax_at_estimate = float("nan")
         self.is_setup = False
         self.timer = None          # optional KernelTimer (operator and transfer launches)
@@ -271,57 +288,103 @@ class GMG:
                   self.nrhs if nrhs is None else nrhs)
 
     # ---------------------------------------------------------------- setup
-    def setup(self, E64: torch.Tensor, E32: torch.Tensor, emax: float, stream) -> None:
-        """Per-SIMP-iteration setup (GalerkinMatFreeGMG.setup, multigrid_v4.py:657-813)."""
+    def setup(self, E64: torch.Tensor, E32: torch.Tensor, emax, stream) -> None:
+        """Per-SIMP-iteration setup (GalerkinMatFreeGMG.setup, multigrid_v4.py:657-813).
+        emax (max E, which decides whether lambda_max is estimated again) is
+        a number or a function returning it; the function is called only
+        once the hierarchy's kernels are queued."""
         self.E64, self.E32 = E64, E32
+        self.join(stream)         # the previous inverse still owns coarse_A
+        self.check()
         lv0 = self.levels[0]
         L.call("tg_diag_inv_hex", lv0.grid, L.ptr(lv0.mask), L.ptr(E64), None, L.ptr(lv0.dinv), stream)
-        prev = self._emax_at_estimate
-        reestimate = np.isnan(prev) or abs(emax - prev) / max(abs(prev), 1e-12) > 0.10
         last_lv = len(self.levels) - 1
+        # the Galerkin chain first, then the coarsest factorisation (which
+        # needs only the last level's matrices), then each level's stencil
+        # and diagonal, which may run beside an asynchronous inverse
         for lv in range(1, len(self.levels)):
             f, c = self.levels[lv - 1], self.levels[lv]
             if lv == 1 and not self.need_l1_mats:
-                pass                                    # applied implicitly (Q8)
+                continue                                # applied implicitly (Q8)
+            if c.mats64 is None:
+                c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
+            m32 = None
+            if lv == 1 and self.l1_from_fine:
+                L.call("tg_galerkin_from_fine", lv0.grid, 1, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
+                       m32, stream)
+            elif lv == 2 and self.l2_from_fine:
+                L.call("tg_galerkin_from_fine", lv0.grid, 2, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
+                       m32, stream)
             else:
-                if c.mats64 is None:
-                    c.mats64 = torch.empty(576 * c.n_elem, dtype=torch.float64, device=self.dev)
-                m32 = None
-                lv0 = self.levels[0]
-                if lv == 1 and self.l1_from_fine:
-                    L.call("tg_galerkin_from_fine", lv0.grid, 1, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
-                           m32, stream)
-                elif lv == 2 and self.l2_from_fine:
-                    L.call("tg_galerkin_from_fine", lv0.grid, 2, L.ptr(E64), L.ptr(self.gwork), L.ptr(c.mats64),
-                           m32, stream)
-                else:
-                    L.call("tg_galerkin_elems", f.grid, c.grid, L.ptr(f.mask), int(not c.consistent),
-                           L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64),
-                           L.ptr(c.mats64), m32, stream)
-            if lv < last_lv:
-                if lv == 1 and self.l1_q8:
-                    L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
-                else:
-                    bf16 = c.kind == L.TG_OP_STENCIL_BF16
-                    if c.stencil is None:
-                        c.stencil = torch.empty(243 * c.nodes, dtype=torch.int16 if bf16 else torch.float32,
-                                                device=self.dev)
-                    L.call("tg_assemble_stencil_bf16" if bf16 else "tg_assemble_stencil", c.grid, L.ptr(c.mask),
-                           L.ptr(c.mats64), L.ptr(c.stencil), stream)
-                    L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
+                L.call("tg_galerkin_elems", f.grid, c.grid, L.ptr(f.mask), int(not c.consistent),
+                       L.ptr(E64) if lv == 1 else None, None if lv == 1 else L.ptr(f.mats64),
+                       L.ptr(c.mats64), m32, stream)
         last = self.levels[-1]
         if self.coarse_bcr:
             L.call("tg_bcr_setup", last.grid, L.ptr(last.mask), L.ptr(last.mats64), L.ptr(self.bcr_work), stream)
         else:
             L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
                    L.ptr(self.coarse_A), stream)
-            L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
+            if self.async_inverse:
+                self._queue_inverse(stream)
+            else:
+                L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
+        for lv in range(1, last_lv):
+            c = self.levels[lv]
+            if lv == 1 and self.l1_q8:
+                L.call("tg_diag_inv_q8", c.grid, L.ptr(c.mask), L.ptr(E64), L.ptr(c.dinv), stream)
+                continue
+            bf16 = c.kind == L.TG_OP_STENCIL_BF16
+            if c.stencil is None:
+                c.stencil = torch.empty(243 * c.nodes, dtype=torch.int16 if bf16 else torch.float32,
+                                        device=self.dev)
+            L.call("tg_assemble_stencil_bf16" if bf16 else "tg_assemble_stencil", c.grid, L.ptr(c.mask),
+                   L.ptr(c.mats64), L.ptr(c.stencil), stream)
+            L.call("tg_diag_inv_stored", c.grid, L.ptr(c.mask), L.ptr(c.mats64), L.ptr(c.dinv), stream)
+        emax = float(emax() if callable(emax) else emax)
+        prev = self._emax_at_estimate
+        reestimate = np.isnan(prev) or abs(emax - prev) / max(abs(prev), 1e-12) > 0.10
         if reestimate:
             for lv in range(last_lv):
                 iters = self.power_iters if lv == 0 else self.coarse_power_iters
                 self.levels[lv].lam = self.estimate_lambda(lv, iters, stream)
             self._emax_at_estimate = emax
         self.is_setup = True
+
+    def _queue_inverse(self, stream) -> None:
+        """coarse_A^-1 on the side stream once the dense matrix is assembled
+        on `stream`; its status is copied to the host behind it."""
+        self._ev_dense.record(self._torch_stream(stream))
+        self._side.wait_event(self._ev_dense)
+        L.call("tg_dense_spd_inverse_async", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self._inv_info),
+               self._side.cuda_stream)
+        with torch.cuda.stream(self._side):
+            self._rb_info.start(self._inv_info)
+        self._ev_inv.record(self._side)
+        self._join_pending = self._check_pending = True
+
+    def _torch_stream(self, handle) -> torch.cuda.Stream:
+        cur = torch.cuda.current_stream(self.dev)
+        return cur if cur.cuda_stream == (handle or 0) else torch.cuda.ExternalStream(handle, device=self.dev)
+
+    def join(self, stream) -> None:
+        """Make `stream` wait for the asynchronous coarsest inverse (once per
+        setup). Called before anything on `stream` reads or rewrites
+        coarse_A, outside graph capture."""
+        if self._join_pending:
+            self._torch_stream(stream).wait_event(self._ev_inv)
+            self._join_pending = False
+
+    def check(self, block: bool = True) -> None:
+        """Raise if the last asynchronous coarsest inverse met a pivot <= 0
+        (as tg_dense_spd_inverse returns TG_ERR_NOT_SPD). Waits for the
+        inverse; with block=False it returns at once while the inverse is
+        still running."""
+        if self._check_pending and (block or self._rb_info.event.query()):
+            self._check_pending = False
+            info = int(self._rb_info.wait()[0])
+            if info != 0:
+                raise L.TopoHipError(f"dense inverse: coarsest matrix not SPD (pivot {info})")
 
     def estimate_lambda(self, lv: int, iters: int, stream) -> float:
         """lambda_max(D^-1 A) by power iteration in the D inner product
@@ -341,7 +404,7 @@ class GMG:
         for _ in range(iters):
             self._launch(lv, self.op(lv, accurate=True), v, kv, ep, stream, nrhs=1)
             L.call("tg_power_step32", Lv.n, L.ptr(v), L.ptr(kv), L.ptr(Lv.dinv), L.ptr(s[1:2]), stream)
-        vkv, vdv = s[0:2].tolist()
+        vkv, vdv = self._rb.read(s[0:2])
         lam = vkv / vdv if vdv > 1e-300 and np.isfinite(vkv / vdv) else 1.0
         return max(lam, 1e-6)
 
@@ -481,12 +544,14 @@ class GMG:
         apply."""
         if not self.is_setup:
             raise RuntimeError("GMG.setup() must be called before apply()")
+        self.join(stream)
         return self._vcycle(0, r, stream)
 
     # apply() in two parts, so a caller can queue the first while it waits
     # on the host for something else: begin() writes only the hierarchy's own
     # buffers (level-0 pre-smoothing and the restriction to level 1), so it
-    # may be issued speculatively and abandoned.
+    # may be issued speculatively and abandoned. begin() does not touch the
+    # coarsest level, so it may run beside the asynchronous inverse.
     def begin(self, r, stream) -> None:
         if not self.is_setup:
             raise RuntimeError("GMG.setup() must be called before apply()")
@@ -494,4 +559,5 @@ class GMG:
 
     def finish(self, stream):
         """The rest of the cycle begin() started; returns z as apply() does."""
+        self.join(stream)
         return self._vcycle_up(0, stream)

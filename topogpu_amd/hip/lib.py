@@ -69,6 +69,7 @@ _SIGS = {
     "tg_assemble_stencil": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_assemble_stencil_bf16": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],
     "tg_dense_spd_inverse": [_vp, _i32, _vp],
+    "tg_dense_spd_inverse_async": [_vp, _i32, _vp, _vp],
     "tg_coarsest_apply": [_vp, _i32, _vp, _i64, _vp, _vp, _vp],
     "tg_bcr_size": [C.POINTER(TgGrid), _vp],
     "tg_bcr_setup": [C.POINTER(TgGrid), _vp, _vp, _vp, _vp],

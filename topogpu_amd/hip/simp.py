@@ -21,6 +21,7 @@ import torch
 
 from .. import fem
 from . import lib as L
+from .readback import Readback
 from .state import HipStateSolver
 
 E0, EMIN = 1.0, 1.0e-9
@@ -91,6 +92,8 @@ class HipSIMP:
         self.hi = torch.zeros(self.n_elem, **f64)
         self.oc_state = torch.zeros(max(L.OC_STATE_LEN, L.OC_MEAN_STATE_LEN), **f64)
         self.scal = torch.zeros(4, **f64)
+        self._rb = Readback(4)            # OC decisions, step count and change
+        self._rb_emax = Readback(1)
 
     def set_rmin(self, rmin: float) -> None:
         """(Re)build the density filter for radius rmin: its offsets and
@@ -145,13 +148,20 @@ class HipSIMP:
         """Set moduli, build the hierarchy and solve; returns (compliance,
         stats). A penalty change above 0.5 since the last solve drops the
         warm start (solver_v4.py:162-164)."""
+        stats = self._solve()
+        return float(self.weights_host @ self.fea.compliances()), stats
+
+    def _solve(self):
+        """Moduli to the hierarchy, then the state solve. max E reaches the
+        host through a copy queued before the hierarchy's setup, which
+        needs it only at its end (whether to re-estimate lambda_max), so the
+        host does not wait on the GPU for it."""
         if self._prev_penal is not None and abs(self.penal - self._prev_penal) > 0.5:
             self.fea.reset_warm_start()
         self._prev_penal = self.penal
-        emax = float(self.E64.max().item())
-        self.fea.set_modulus(self.E64, self.E32, emax)
-        stats = self.fea.solve()
-        return float(self.weights_host @ self.fea.compliances()), stats
+        emax = self._rb_emax.start(self.E64.max().reshape(1))
+        self.fea.set_modulus(self.E64, self.E32, lambda: float(emax.wait()[0]))
+        return self.fea.solve()
 
     def sensitivities(self) -> None:
         """dc = H^T(dc_phys * dh), dv = H^T dh (pub_simp_solver.py:361-371);
@@ -178,12 +188,12 @@ class HipSIMP:
                 L.call("tg_oc_mean_step", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.volfrac, self.move,
                        self.rho_min, L.ptr(self.oc_state), st)
             passes += 4
-            if float(self.oc_state[7].item()):
+            if self._rb.read(self.oc_state[7:8])[0]:
                 break
         L.call("tg_oc_mean_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.move, self.rho_min,
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
-        change = self._apply_passive(self.rho_new)
-        n_steps = int(self.oc_state[3].item())
+        n_steps, change = self._rb.read(self.oc_state[3:4], self._apply_passive(self.rho_new))
+        n_steps, change = int(n_steps), float(change)
         self.rho, self.rho_new = self.rho_new, self.rho
         return change, n_steps
 
@@ -207,34 +217,38 @@ class HipSIMP:
                        L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
                        self.volfrac, use_h, L.ptr(self.oc_state), st)
             passes += chunk
-            if float(self.oc_state[3].item()):
+            if self._rb.read(self.oc_state[3:4])[0]:
                 break
         L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
-        change = self._apply_passive(self.rho_new)
-        n_steps = int(self.oc_state[4].item())
+        n_steps, change = self._rb.read(self.oc_state[4:5], self._apply_passive(self.rho_new))
+        n_steps, change = int(n_steps), float(change)
         self.rho, self.rho_new = self.rho_new, self.rho
         return change, n_steps
 
     def _passive_lo(self) -> float:
         return self.rho_min if self.formulation == "solverv4" else 1.0e-3
 
-    def _apply_passive(self, rho_new) -> float:
+    def _apply_passive(self, rho_new) -> torch.Tensor:
         """Void elements to the lower density bound, solid ones to 1
         (apply_passive_mask, bc_generator.py:454-470; rho_min = 1e-3 there,
-        the filtered OC's bound). Returns max |rho_new - rho| of the design
-        the next iteration uses: the finalize kernel's value, or, with
-        passive elements, recomputed after the clamp."""
+        the filtered OC's bound). Returns, on the device (one element),
+        max |rho_new - rho| of the design the next iteration uses: the
+        finalize kernel's value, or, with passive elements, recomputed after
+        the clamp."""
         if self.passive is None:
-            return float(self.scal[0].item())
+            return self.scal[0:1]
         L.call("tg_passive64", self.n_elem, L.ptr(self.passive), L.ptr(rho_new), self._passive_lo(), self.stream)
-        return float((rho_new - self.rho).abs().max().item())
+        return (rho_new - self.rho).abs().max().reshape(1)
 
     def step(self) -> StepResult:
         self.physical()
-        c, stats = self.state()
+        stats = self._solve()
+        # the compliance is read once the OC has needed the host anyway
+        comp = self.fea.compliances_start()
         self.sensitivities()
         change, n_oc = self.oc()
+        c = float(self.weights_host @ comp.wait())
         return StepResult(c, change, stats.iters, stats.restarts, stats.rel_residual, stats.converged, n_oc)
 
     def evaluate(self):

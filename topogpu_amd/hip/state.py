@@ -31,6 +31,7 @@ from .. import fem
 from . import graph
 from . import lib as L
 from .gmg import GMG, launch_op
+from .readback import Readback
 
 # precision -> (Krylov operator, level-0 smoother operator, level-0
 #               post-correction operator, level-1 post-correction operator,
@@ -152,8 +153,11 @@ class HipStateSolver:
         # device scalars, one row of nrhs per quantity:
         # 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 compliance
         self.s = torch.zeros(8, self.nrhs, dtype=torch.float64, device=self.dev)
-        self._rr_host = torch.zeros(self.nrhs, dtype=torch.float64, pin_memory=True)
-        self._rr_ready = torch.cuda.Event()
+        # host reads of device scalars through pinned memory: r.r in the
+        # Krylov loop, the other residual norms and sums, the compliances
+        self._rb_rr = Readback(self.nrhs)
+        self._rb = Readback(max(8 * self.nrhs, int(restart) + 2))
+        self._rb_c = Readback(self.nrhs)
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
@@ -187,10 +191,14 @@ class HipStateSolver:
         the nrhs norms ||r64||."""
         L.call("tg_apply64_b", self.grid, L.ptr(self.mask), L.ptr(self.E64), self.nrhs, L.ptr(self.u),
                L.ptr(self.F), L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4]), self.stream)
-        return np.sqrt(np.maximum(self.s[4].cpu().numpy(), 0.0))
+        rr = self._rb.read(self.s[4])
+        self.gmg.check(block=False)
+        return np.sqrt(np.maximum(rr, 0.0))
 
     def solve(self) -> SolveStats:
         stats = self._solve_fgmres() if self.outer_solver == "fgmres" else self._solve_pcg()
+        self.gmg.join(self.stream)
+        self.gmg.check()
         if self.probe:
             self.last_multigrid_quality = self.probe_quality()
         return stats
@@ -212,7 +220,7 @@ class HipStateSolver:
         L.call("tg_cheb_init32_b", n, nr, L.ptr(self.r), L.ptr(self.gmg.levels[0].dinv), 1.0, L.ptr(self.p),
                L.ptr(self.q), None, st)                                                      # p = D^-1 F
         L.call("tg_dot32_b", n, nr, L.ptr(self.p), L.ptr(self.p), None, L.ptr(ps[2]), None, st)
-        fz, zz, jj = ps.cpu().numpy()
+        fz, zz, jj = self._rb.read(ps).reshape(3, nr)
         zn, jn = np.sqrt(np.maximum(zz, 0.0)), np.sqrt(np.maximum(jj, 0.0))
         q = {"finite": np.isfinite(zz), "pd": fz > 0.0, "z_norm": zn, "jacobi_norm": jn,
              "z_over_jacobi": zn / np.maximum(jn, 1e-30)}
@@ -337,7 +345,7 @@ class HipStateSolver:
                     L.call("tg_axpy64", n, L.ptr(w), L.ptr(V[i]), None, L.ptr(h[i:i + 1]), -1.0, 1, st)
                 L.call("tg_dot64_b", n, 1, L.ptr(w), L.ptr(w), L.ptr(h[j + 1:j + 2]), st)
                 L.call("tg_normalize64", n, L.ptr(w), L.ptr(h[j + 1:j + 2]), L.ptr(V[j + 1]), L.ptr(self.r), st)
-                col = h[:j + 2].cpu().numpy()
+                col = self._rb.read(h[:j + 2])
                 H[:j + 1, j] = col[:j + 1]
                 h_norm = float(np.sqrt(max(col[j + 1], 0.0)))
                 H[j + 1, j] = h_norm
@@ -376,16 +384,15 @@ class HipStateSolver:
         V-cycle's first part (gmg.begin, which touches only the hierarchy's
         buffers) is queued behind the copy before the host waits, so the GPU
         is not idle while the host reads the value and queues the rest."""
-        if not speculate:
-            return self.s[2].cpu().numpy()
-        self._rr_host.copy_(self.s[2], non_blocking=True)
-        self._rr_ready.record()
-        if self._replaying():
-            self._replay("begin", st)
-        else:
-            self.gmg.begin(self.r, st)
-        self._rr_ready.synchronize()
-        return self._rr_host.numpy().copy()
+        self._rb_rr.start(self.s[2])
+        if speculate:
+            if self._replaying():
+                self._replay("begin", st)
+            else:
+                self.gmg.begin(self.r, st)
+        rr = self._rb_rr.wait()
+        self.gmg.check(block=False)
+        return rr
 
     def _issue_iteration(self, first: bool, st, begun: bool = False) -> None:
         """One flexible-PCG iteration on stream st: z = M r, then
@@ -417,6 +424,8 @@ class HipStateSolver:
         part; "finish" / "finish_first": the rest of a later / first
         iteration), capturing it again when what it baked in changed (the
         hierarchy's coefficients and buffers, the modulus buffer)."""
+        if name != "begin":
+            self.gmg.join(st)     # outside the capture: the wait is not recorded
         key = (self.gmg.graph_key(), L.ptr(self.E32))
         g = self._graphs.get(name)
         if g is None or g.key != key:
@@ -440,10 +449,16 @@ class HipStateSolver:
             self._replay("begin", st)
         self._replay("finish_first" if first else "finish", st)
 
+    def compliances_start(self) -> Readback:
+        """Queue F_k . u_k in FP64 for every load case (solver_v4.py:244) and
+        its copy to the host; the returned reader's wait() gives the values,
+        so the host may queue further work before it blocks."""
+        L.call("tg_dot64_b", self.n, self.nrhs, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5]), self.stream)
+        return self._rb_c.start(self.s[5])
+
     def compliances(self) -> np.ndarray:
         """F_k . u_k in FP64 for every load case (solver_v4.py:244)."""
-        L.call("tg_dot64_b", self.n, self.nrhs, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5]), self.stream)
-        return self.s[5].cpu().numpy()
+        return self.compliances_start().wait()
 
     def compliance(self) -> float:
         """F . u in FP64 (solver_v4.py:244); the sum over load cases of a batch."""
