@@ -181,15 +181,21 @@ class HipSIMP:
         st = self.stream
         L.call("tg_oc_mean_prepare", L.ptr(self.oc_state), st)
         # one doubling pass, then six bisection steps per pass (80 at most);
-        # calls after the stop return at once, so a few are queued per check
-        passes = 0
-        while passes < 40:
-            for _ in range(4):
+        # calls after the stop return at once, so the next four are queued
+        # before the host reads whether the last four stopped it
+        def queue(k):
+            for _ in range(k):
                 L.call("tg_oc_mean_step", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.volfrac, self.move,
                        self.rho_min, L.ptr(self.oc_state), st)
-            passes += 4
-            if self._rb.read(self.oc_state[7:8])[0]:
+        queue(4)
+        passes = 4
+        while True:
+            self._rb.start(self.oc_state[7:8])
+            ahead = 4 if passes < 40 else 0
+            queue(ahead)
+            if self._rb.wait()[0] or passes >= 40:
                 break
+            passes += ahead
         L.call("tg_oc_mean_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.dc), self.move, self.rho_min,
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
         n_steps, change = self._rb.read(self.oc_state[3:4], self._apply_passive(self.rho_new))
@@ -208,17 +214,23 @@ class HipSIMP:
         use_h = int(self.use_heaviside and self.beta > 1e-6)
         # each call evaluates one batch of multipliers and advances the
         # bisection by at least two steps (usually far more); calls after
-        # convergence return at once, so a few are queued between checks
-        passes = 0
-        while passes < (max_steps + 1) // 2:
-            chunk = 5 if passes == 0 else 2
-            for _ in range(chunk):
+        # convergence return at once, so five are queued first and then two
+        # more before each read of whether those before them converged
+        def queue(k):
+            for _ in range(k):
                 L.call("tg_oc_bisect_step", self.grid, self.m, self.offs.ctypes.data, self.w.ctypes.data,
                        L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
                        self.volfrac, use_h, L.ptr(self.oc_state), st)
-            passes += chunk
-            if self._rb.read(self.oc_state[3:4])[0]:
+        limit = (max_steps + 1) // 2
+        queue(5)
+        passes = 5
+        while True:
+            self._rb.start(self.oc_state[3:4])
+            ahead = 2 if passes < limit else 0
+            queue(ahead)
+            if self._rb.wait()[0] or passes >= limit:
                 break
+            passes += ahead
         L.call("tg_oc_finalize", self.n_elem, L.ptr(self.rho), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi),
                L.ptr(self.oc_state), L.ptr(self.rho_new), L.ptr(self.scal[0:1]), st)
         n_steps, change = self._rb.read(self.oc_state[4:5], self._apply_passive(self.rho_new))

@@ -186,12 +186,17 @@ class HipStateSolver:
         self.E64, self.E32 = E64, E32
         self.gmg.setup(E64, E32, emax, self.stream)
 
-    def residual64(self) -> np.ndarray:
+    def residual64(self, speculate: bool = False) -> np.ndarray:
         """r64 = F - K u (FP64), r = float(r64) for every load case; returns
-        the nrhs norms ||r64||."""
+        the nrhs norms ||r64||. With speculate, the V-cycle's first part on
+        the new r is queued before the host waits for the norms (the next
+        call of _iteration may then pass begun=True)."""
         L.call("tg_apply64_b", self.grid, L.ptr(self.mask), L.ptr(self.E64), self.nrhs, L.ptr(self.u),
                L.ptr(self.F), L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4]), self.stream)
-        rr = self._rb.read(self.s[4])
+        self._rb.start(self.s[4])
+        if speculate:
+            self._begin(self.stream)
+        rr = self._rb.wait()
         self.gmg.check(block=False)
         return np.sqrt(np.maximum(rr, 0.0))
 
@@ -250,7 +255,10 @@ class HipStateSolver:
             L.call("tg_fill64", nr * n, L.ptr(self.u), 0.0, st)
         target = self.tol * self.bnorms
         its = restarts = 0
-        rnorm = self.residual64()
+        # each FP64 residual queues the next restart's first V-cycle part
+        # before the host reads it (wasted, harmlessly, after the last one)
+        spec = self.speculate
+        rnorm = self.residual64(spec)
         restarts += 1
         if not np.all(np.isfinite(rnorm)):
             raise FloatingPointError("non-finite residual at the start of the state solve")
@@ -260,12 +268,12 @@ class HipStateSolver:
             # than zero: drop it before spending iterations on it
             for k in np.flatnonzero(far):
                 L.call("tg_fill64", n, L.ptr(self._slice(self.u, k)), 0.0, st)
-            rnorm = self.residual64()
+            rnorm = self.residual64(spec)
             restarts += 1
         s = self.s
         while np.any(rnorm > target) and its < self.max_krylov and restarts <= self.max_restarts:
             inner_target = np.maximum(self.inner_rtol * rnorm, 0.5 * target)
-            self._iteration(True, st)
+            self._iteration(True, st, begun=spec)
             its += 1
             while True:
                 # the next V-cycle's first part runs while the host waits for
@@ -279,7 +287,7 @@ class HipStateSolver:
                 self._iteration(False, st, begun=speculate)
                 its += 1
             L.call("tg_axpy_f32_to_f64", nr * n, L.ptr(self.u), L.ptr(self.d), st)
-            rnorm = self.residual64()
+            rnorm = self.residual64(spec)
             restarts += 1
         rel = rnorm / np.maximum(self.bnorms, 1e-300)
         self.last = SolveStats(its, restarts, bool(np.all(rnorm <= target)), float(rel.max()))
@@ -386,13 +394,18 @@ class HipStateSolver:
         is not idle while the host reads the value and queues the rest."""
         self._rb_rr.start(self.s[2])
         if speculate:
-            if self._replaying():
-                self._replay("begin", st)
-            else:
-                self.gmg.begin(self.r, st)
+            self._begin(st)
         rr = self._rb_rr.wait()
         self.gmg.check(block=False)
         return rr
+
+    def _begin(self, st) -> None:
+        """The V-cycle's first part on self.r (gmg.begin, which writes only
+        the hierarchy's buffers), replayed or launched."""
+        if self._replaying():
+            self._replay("begin", st)
+        else:
+            self.gmg.begin(self.r, st)
 
     def _issue_iteration(self, first: bool, st, begun: bool = False) -> None:
         """One flexible-PCG iteration on stream st: z = M r, then
