@@ -138,65 +138,97 @@ __global__ void __launch_bounds__(256) k_restrict(Grid f, Grid c, const uint8_t*
   o[2] = (m & 4u) ? acc[2] : 0.f;
 }
 
-// One block per coarse element: K_C = sum_f (M_f P_f)^T K_f (M_f P_f).
+// One block per coarse element: K_C = sum_f (M_f P_f)^T K_f (M_f P_f) over
+// its children f in (x, y, z) loop order (two per axis, three at the end of
+// an odd fine extent). Up to kGalChildren children per round: their
+// matrices, local prolongations and masks are loaded together (one memory
+// round trip instead of one per child), each thread turns one row of one
+// child's K into the same row of T = K (M P) in place, and every coarse entry
+// adds the children's (M P)^T T in child order -- the sums and their order
+// of a loop over one child at a time.
+constexpr int kGalChildren = 8;
+
 __global__ void __launch_bounds__(256) k_galerkin(Grid f, Grid c, const uint8_t* __restrict__ mask_f, int use_mask,
                                                  const double* __restrict__ fE, const double* __restrict__ fmats,
                                                  double* __restrict__ out64, float* __restrict__ out32) {
-  __shared__ double sK[576];
-  __shared__ double sT[576];
-  __shared__ double sP[8][8];   // local P: fine corner x coarse corner
-  __shared__ double sM[24];
+  __shared__ double sK[kGalChildren][576];    // K, then T, of each child of the round
+  __shared__ double sP[kGalChildren][8][8];   // local P: fine corner x coarse corner
+  __shared__ double sM[kGalChildren][24];
   const long long C = blockIdx.x;
   const int CK = (int)(C % c.nz), CJ = (int)((C / c.nz) % c.ny), CI = (int)(C / ((long long)c.nz * c.ny));
   // children ranges per axis
   const int fx0 = 2 * CI, fx1 = (CI == c.nx - 1) ? f.nx - 1 : 2 * CI + 1;
   const int fy0 = 2 * CJ, fy1 = (CJ == c.ny - 1) ? f.ny - 1 : 2 * CJ + 1;
   const int fz0 = 2 * CK, fz1 = (CK == c.nz - 1) ? f.nz - 1 : 2 * CK + 1;
+  const int cny = fy1 - fy0 + 1, cnz = fz1 - fz0 + 1;
+  const int nch = (fx1 - fx0 + 1) * cny * cnz;
+  auto child = [&](int id, int& fi, int& fj, int& fk) {
+    fk = fz0 + id % cnz;
+    fj = fy0 + (id / cnz) % cny;
+    fi = fx0 + id / (cnz * cny);
+  };
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int fi = fx0; fi <= fx1; ++fi)
-    for (int fj = fy0; fj <= fy1; ++fj)
-      for (int fk = fz0; fk <= fz1; ++fk) {
-        const long long fe = f.elem(fi, fj, fk);
-        for (int t = threadIdx.x; t < 576; t += blockDim.x)
-          sK[t] = fmats ? fmats[576 * fe + t] : fE[fe] * c_ke64[t];
-        if (threadIdx.x < 64) {
-          const int a = threadIdx.x >> 3, b = threadIdx.x & 7;   // fine corner a, coarse corner b
-          sP[a][b] = axis_w(fi + corner_x(a), CI + corner_x(b), c.nx) *
+  for (int c0 = 0; c0 < nch; c0 += kGalChildren) {
+    const int cn = nch - c0 < kGalChildren ? nch - c0 : kGalChildren;
+    for (int e = threadIdx.x; e < cn * 576; e += blockDim.x) {
+      const int ch = e / 576, t = e - ch * 576;
+      int fi, fj, fk;
+      child(c0 + ch, fi, fj, fk);
+      const long long fe = f.elem(fi, fj, fk);
+      sK[ch][t] = fmats ? fmats[576 * fe + t] : fE[fe] * c_ke64[t];
+    }
+    for (int e = threadIdx.x; e < cn * 64; e += blockDim.x) {
+      const int ch = e >> 6, a = (e >> 3) & 7, b = e & 7;   // fine corner a, coarse corner b
+      int fi, fj, fk;
+      child(c0 + ch, fi, fj, fk);
+      sP[ch][a][b] = axis_w(fi + corner_x(a), CI + corner_x(b), c.nx) *
                      axis_w(fj + corner_y(a), CJ + corner_y(b), c.ny) *
                      axis_w(fk + corner_z(a), CK + corner_z(b), c.nz);
-        }
-        if (threadIdx.x < 24) {
-          const int a = threadIdx.x / 3, q = threadIdx.x % 3;
-          double mv = 1.0;
-          if (use_mask) {
-            const long long n = f.node(fi + corner_x(a), fj + corner_y(a), fk + corner_z(a));
-            mv = (mask_f[n] >> q & 1u) ? 1.0 : 0.0;
-          }
-          sM[threadIdx.x] = mv;
-        }
-        __syncthreads();
-        // T[a][B] = sum_b K[a][b] (MP)[b][B]
-        for (int t = threadIdx.x; t < 576; t += blockDim.x) {
-          const int a = t / 24, B = t % 24, q = B % 3;
+    }
+    for (int e = threadIdx.x; e < cn * 24; e += blockDim.x) {
+      const int ch = e / 24, l = e - ch * 24, a = l / 3, q = l % 3;
+      double mv = 1.0;
+      if (use_mask) {
+        int fi, fj, fk;
+        child(c0 + ch, fi, fj, fk);
+        const long long n = f.node(fi + corner_x(a), fj + corner_y(a), fk + corner_z(a));
+        mv = (mask_f[n] >> q & 1u) ? 1.0 : 0.0;
+      }
+      sM[ch][l] = mv;
+    }
+    __syncthreads();
+    // T[a][B] = sum_b K[a][b] (MP)[b][B], one row of one child per thread
+    if (threadIdx.x < cn * 24) {
+      const int ch = threadIdx.x / 24, a = threadIdx.x % 24;
+      double kr[24];
+#pragma unroll
+      for (int b = 0; b < 24; ++b) kr[b] = sK[ch][a * 24 + b];
+#pragma unroll 1
+      for (int Bn = 0; Bn < 8; ++Bn)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
           double s = 0.0;
 #pragma unroll
-          for (int nb = 0; nb < 8; ++nb) s += sK[a * 24 + 3 * nb + q] * sM[3 * nb + q] * sP[nb][B / 3];
-          sT[t] = s;
+          for (int nb = 0; nb < 8; ++nb) s += kr[3 * nb + q] * sM[ch][3 * nb + q] * sP[ch][nb][Bn];
+          sK[ch][a * 24 + 3 * Bn + q] = s;
         }
-        __syncthreads();
-        // K_C[A][B] += sum_a (MP)[a][A] T[a][B]
-        for (int r = 0; r < 3; ++r) {
-          const int t = threadIdx.x + r * 256;
-          if (t < 576) {
-            const int A = t / 24, B = t % 24, q = A % 3;
-            double s = 0.0;
+    }
+    __syncthreads();
+    // K_C[A][B] += sum_a (MP)[a][A] T[a][B], child by child
+    for (int r = 0; r < 3; ++r) {
+      const int t = threadIdx.x + r * 256;
+      if (t < 576) {
+        const int A = t / 24, B = t % 24, q = A % 3;
+        for (int ch = 0; ch < cn; ++ch) {
+          double s = 0.0;
 #pragma unroll
-            for (int na = 0; na < 8; ++na) s += sP[na][A / 3] * sM[3 * na + q] * sT[(3 * na + q) * 24 + B];
-            acc[r] += s;
-          }
+          for (int na = 0; na < 8; ++na) s += sP[ch][na][A / 3] * sM[ch][3 * na + q] * sK[ch][(3 * na + q) * 24 + B];
+          acc[r] += s;
         }
-        __syncthreads();
       }
+    }
+    __syncthreads();
+  }
   for (int r = 0; r < 3; ++r) {
     const int t = threadIdx.x + r * 256;
     if (t < 576) {
