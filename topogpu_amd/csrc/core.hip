@@ -124,6 +124,34 @@ __global__ void k_fill64(long long n, double* x, double v) {
 // scalars are arrays indexed by r. The grid-stride loops are unrolled (a
 // thread has about four elements) so that their loads are in flight
 // together.
+// The FP32 vector kernels below give each thread groups of four consecutive
+// entries counted from the start of the slice (groups t, t + S, ... for
+// thread t of S), each group loaded as one float4 when every vector of the
+// slice starts on a 16-byte boundary -- four times the bytes in flight per
+// load of one float -- and as four floats otherwise (a right-hand side at an
+// odd offset); the last group of n % 4 entries goes entry by entry. The
+// entries of a thread and their order thus depend only on n and the launch
+// shape, not on alignment: a slice of a batch gives the bits of the same
+// slice alone.
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+__device__ __forceinline__ float4 ld4(const float* p, long long g, bool vec) {
+  if (vec) return reinterpret_cast<const float4*>(p)[g];
+  const float* q = p + 4 * g;
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ void st4(float* p, long long g, bool vec, float4 v) {
+  if (vec) {
+    reinterpret_cast<float4*>(p)[g] = v;
+  } else {
+    float* q = p + 4 * g;
+    q[0] = v.x;
+    q[1] = v.y;
+    q[2] = v.z;
+    q[3] = v.w;
+  }
+}
+
 __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a, const float* b,
                                                     const float* c, Workspace ws) {
   const long long o = blockIdx.y * n;
@@ -131,13 +159,31 @@ __global__ void __launch_bounds__(kVecBlock) k_dot32(long long n, const float* a
   b += o;
   if (c) c += o;
   double v[2] = {0.0, 0.0};
-  #pragma unroll 4
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double ai = a[i];
-    v[0] += ai * (double)b[i];
-    if (c) v[1] += ai * (double)c[i];
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+  const bool vec = aligned16(a) && aligned16(b) && (!c || aligned16(c));
+  const long long n4 = n >> 2;
+  #pragma unroll 2
+  for (long long g = t0; g < n4; g += stride) {
+    const float4 x = ld4(a, g, vec), y = ld4(b, g, vec);
+    const double ax = x.x, ay = x.y, az = x.z, aw = x.w;
+    v[0] += ax * (double)y.x;
+    v[0] += ay * (double)y.y;
+    v[0] += az * (double)y.z;
+    v[0] += aw * (double)y.w;
+    if (c) {
+      const float4 z = ld4(c, g, vec);
+      v[1] += ax * (double)z.x;
+      v[1] += ay * (double)z.y;
+      v[1] += az * (double)z.z;
+      v[1] += aw * (double)z.w;
+    }
   }
+  if (n4 % stride == t0)   // the thread whose next group is the partial one
+    for (long long i = 4 * n4; i < n; ++i) {
+      const double ai = a[i];
+      v[0] += ai * (double)b[i];
+      if (c) v[1] += ai * (double)c[i];
+    }
   block_partials<2>(v, ws);
 }
 
@@ -198,14 +244,29 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_xr(long long n, float* x, flo
   const double den = pq[k];
   const float alpha = (float)(den != 0.0 ? rz[k] / den : 0.0);
   double v[1] = {0.0};
-  #pragma unroll 4
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    x[i] = fmaf(alpha, p[i], x[i]);
-    const float ri = fmaf(-alpha, q[i], r[i]);
-    r[i] = ri;
-    v[0] += (double)ri * (double)ri;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+  const bool vec = aligned16(x) && aligned16(r) && aligned16(p) && aligned16(q);
+  const long long n4 = n >> 2;
+  #pragma unroll 2
+  for (long long g = t0; g < n4; g += stride) {
+    const float4 pi = ld4(p, g, vec), qi = ld4(q, g, vec), xi = ld4(x, g, vec), r0 = ld4(r, g, vec);
+    st4(x, g, vec, make_float4(fmaf(alpha, pi.x, xi.x), fmaf(alpha, pi.y, xi.y), fmaf(alpha, pi.z, xi.z),
+                               fmaf(alpha, pi.w, xi.w)));
+    const float4 ri = make_float4(fmaf(-alpha, qi.x, r0.x), fmaf(-alpha, qi.y, r0.y), fmaf(-alpha, qi.z, r0.z),
+                                  fmaf(-alpha, qi.w, r0.w));
+    st4(r, g, vec, ri);
+    v[0] += (double)ri.x * (double)ri.x;
+    v[0] += (double)ri.y * (double)ri.y;
+    v[0] += (double)ri.z * (double)ri.z;
+    v[0] += (double)ri.w * (double)ri.w;
   }
+  if (n4 % stride == t0)
+    for (long long i = 4 * n4; i < n; ++i) {
+      x[i] = fmaf(alpha, p[i], x[i]);
+      const float ri = fmaf(-alpha, q[i], r[i]);
+      r[i] = ri;
+      v[0] += (double)ri * (double)ri;
+    }
   block_partials<1>(v, ws);
 }
 
@@ -216,10 +277,17 @@ __global__ void k_pcg_dir(long long n, float* p, const float* z, const double* z
   z += o;
   const double den = pq[k];
   const float beta = (float)(den != 0.0 ? -zq[k] / den : 0.0);
-  #pragma unroll 4
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    p[i] = fmaf(beta, p[i], z[i]);
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+  const bool vec = aligned16(p) && aligned16(z);
+  const long long n4 = n >> 2;
+  #pragma unroll 2
+  for (long long g = t0; g < n4; g += stride) {
+    const float4 pi = ld4(p, g, vec), zi = ld4(z, g, vec);
+    st4(p, g, vec, make_float4(fmaf(beta, pi.x, zi.x), fmaf(beta, pi.y, zi.y), fmaf(beta, pi.z, zi.z),
+                               fmaf(beta, pi.w, zi.w)));
+  }
+  if (n4 % stride == t0)
+    for (long long i = 4 * n4; i < n; ++i) p[i] = fmaf(beta, p[i], z[i]);
 }
 
 // dinv is shared by all right-hand sides (one operator).
