@@ -160,6 +160,7 @@ def simp_bench(precision: str, steps: int, warmup: int, n_levels=None, saturated
     dt = time.perf_counter() - t0
     gc.unfreeze()
     eng.fea.gmg.timer = None
+    counter.flush()
     for i, r in enumerate(rows):
         if i < 3 or i >= steps - 3 or (i + 1) % 10 == 0:
             _log(f"{precision} step {i + 1}/{steps}: C={r.compliance:.6g} its={r.iters} restarts={r.restarts} "

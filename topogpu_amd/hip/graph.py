@@ -99,8 +99,18 @@ def capture(key, issue, owner, device) -> LaunchGraph:
 
 
 def account(timer, graph: LaunchGraph) -> None:
-    """Add one replay's accounted launches to a KernelTimer's counters."""
+    """Add one replay's accounted launches to a KernelTimer's counters (as a
+    pending replay when the timer keeps them, see KernelTimer.flush)."""
     if timer is None:
+        return
+    pending = getattr(timer, "graphs", None)
+    if pending is not None:
+        n = pending.get(graph)
+        if n is None:
+            # the graph's launches, with this replay's operands, are the
+            # ones a later timing of each key relaunches
+            timer.replay.update(graph.replay)
+        pending[graph] = 1 if n is None else n + 1
         return
     for k, n in graph.counts.items():
         timer.counts[k] += n

@@ -9,6 +9,9 @@ region uses ``events=False``: launches and algorithmic bytes are counted per
 key is kept as a closure (``replay``; called with no argument it launches on
 its original stream, called with a stream handle on that stream), so that its
 duration can be measured afterwards by launching it again back to back.
+A replayed launch graph is counted as one increment (``graphs``); its
+launches join ``counts`` and ``bytes`` when ``flush`` is called, so the
+accounting costs the timed loop one dictionary update per replay.
 """
 
 from __future__ import annotations
@@ -25,6 +28,15 @@ class KernelTimer:
         self.counts = defaultdict(int)            # (name, mode) -> launches
         self.bytes = defaultdict(float)           # (name, mode) -> bytes
         self.replay = {}                          # (name, mode) -> (fn, bytes of one launch)
+        self.graphs = {}                          # LaunchGraph -> replays not yet in counts
+
+    def flush(self) -> None:
+        """Move the pending graph replays into counts and bytes."""
+        for g, reps in self.graphs.items():
+            for k, n in g.counts.items():
+                self.counts[k] += n * reps
+                self.bytes[k] += g.nbytes.get(k, 0.0) * reps
+        self.graphs.clear()
 
     def launch(self, name: str, nbytes: float, fn, mode: int = -1):
         self.counts[(name, mode)] += 1
@@ -44,6 +56,7 @@ class KernelTimer:
         """Per kernel name: launches, and with events the average duration
         and the achieved algorithmic bandwidth."""
         out = {}
+        self.flush()
         for (name, _), n in self.counts.items():
             out.setdefault(name, {"launches": 0})["launches"] += n
         if self.events:
@@ -62,3 +75,4 @@ class KernelTimer:
         self.counts.clear()
         self.bytes.clear()
         self.replay.clear()
+        self.graphs.clear()
