@@ -290,27 +290,38 @@ __device__ __forceinline__ void write_dinv(long long n, unsigned m, const double
   }
 }
 
-// kind 0: E per element (KE); 1: fine E (Q8 sums); 2: stored FP64 matrices
-__global__ void k_diag(Grid g, const uint8_t* __restrict__ mask, int kind, const double* __restrict__ w,
-                       Grid fine, double* d64, float* d32) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= g.nodes()) return;
-  const int k = (int)(n % g.NZ()), j = (int)((n / g.NZ()) % g.NY()), i = (int)(n / ((long long)g.NZ() * g.NY()));
+// KIND 0: E per element (KE); 1: fine E (Q8 sums); 2: stored FP64 matrices.
+// Each element's weights are loaded once (all eight children's for KIND 1)
+// and the loops are unrolled, so a node's loads are all in flight together;
+// every d[r] adds its terms in (corner, child) order.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_diag(Grid g, const uint8_t* __restrict__ mask, const double* __restrict__ w,
+                                             Grid fine, double* d64, float* d32) {
+  const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;   // nodes < 2^32 (checked by the launcher)
+  if (n >= (unsigned)g.nodes()) return;
+  const unsigned t = n / (unsigned)g.NZ();
+  const int k = (int)(n - t * (unsigned)g.NZ()), j = (int)(t % (unsigned)g.NY()), i = (int)(t / (unsigned)g.NY());
   double d[3] = {0.0, 0.0, 0.0};
+#pragma unroll
   for (int o = 0; o < 8; ++o) {
     const int ei = i - corner_x(o), ej = j - corner_y(o), ek = k - corner_z(o);
     if (ei < 0 || ei >= g.nx || ej < 0 || ej >= g.ny || ek < 0 || ek >= g.nz) continue;
-    for (int r = 0; r < 3; ++r) {
-      const int rr = 3 * o + r;
-      if (kind == 0) {
-        d[r] += w[g.elem(ei, ej, ek)] * c_ke64[rr * 24 + rr];
-      } else if (kind == 1) {
-        for (int s = 0; s < 8; ++s)
-          d[r] += w[fine.elem(2 * ei + (s >> 2 & 1), 2 * ej + (s >> 1 & 1), 2 * ek + (s & 1))] *
-                  (double)c_q8_32[576 * s + rr * 24 + rr];
-      } else {
-        d[r] += w[576 * g.elem(ei, ej, ek) + rr * 24 + rr];
-      }
+    if constexpr (KIND == 0) {
+      const double we = w[g.elem(ei, ej, ek)];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) d[r] += we * c_ke64[(3 * o + r) * 25];
+    } else if constexpr (KIND == 1) {
+      double ws[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) ws[s] = w[fine.elem(2 * ei + (s >> 2 & 1), 2 * ej + (s >> 1 & 1), 2 * ek + (s & 1))];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) d[r] += ws[s] * (double)c_q8_32[576 * s + (3 * o + r) * 25];
+    } else {
+      const double* m = w + 576 * g.elem(ei, ej, ek);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) d[r] += m[(3 * o + r) * 25];
     }
   }
   write_dinv(n, mask[n], d, d64, d32);
@@ -498,8 +509,12 @@ static int launch_diag(const tg_grid* gp, const uint8_t* mask, int kind, const d
   TG_REQUIRE(mask && w && (d64 || d32), "diag: NULL argument");
   const Grid g = to_grid(gp);
   const long long n = g.nodes();
-  hipLaunchKernelGGL(k_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g, mask,
-                     kind, w, fine, d64, d32);
+  TG_REQUIRE(n < (1LL << 32) - 256, "diag: grid of %lld nodes exceeds 32-bit node indices", n);
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  const hipStream_t st = (hipStream_t)stream;
+  if (kind == 0) hipLaunchKernelGGL(k_diag<0>, grid, block, 0, st, g, mask, w, fine, d64, d32);
+  else if (kind == 1) hipLaunchKernelGGL(k_diag<1>, grid, block, 0, st, g, mask, w, fine, d64, d32);
+  else hipLaunchKernelGGL(k_diag<2>, grid, block, 0, st, g, mask, w, fine, d64, d32);
   TG_LAUNCH_CHECK();
   return TG_OK;
 }
