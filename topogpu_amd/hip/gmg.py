@@ -257,10 +257,11 @@ class GMG:
         self.scal = torch.zeros(8, dtype=torch.float64, device=device)
         self._rb = Readback(2)
         # The one-workgroup coarsest inverse occupies one compute unit for
-        # about 160 us at 135 DOFs: it runs on a side stream, beside the
-        # levels' stencil assembly and the start of the solve, and the first
-        # use of the coarsest level waits for it (join); its status reaches
-        # the host with the next read (check).
+        # about 160 us at 135 DOFs: it runs, with the dense assembly before
+        # it, on a side stream, beside the levels' stencil assembly and the
+        # start of the solve, and the first use of the coarsest level waits
+        # for it (join); its status reaches the host with the next read
+        # (check).
         self.async_inverse = not self.coarse_bcr and self.n_coarse <= ASYNC_INVERSE_MAX
         if self.async_inverse:
             self._side = torch.cuda.Stream(device)
@@ -322,13 +323,12 @@ class GMG:
         last = self.levels[-1]
         if self.coarse_bcr:
             L.call("tg_bcr_setup", last.grid, L.ptr(last.mask), L.ptr(last.mats64), L.ptr(self.bcr_work), stream)
+        elif self.async_inverse:
+            self._queue_inverse(stream)
         else:
             L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
                    L.ptr(self.coarse_A), stream)
-            if self.async_inverse:
-                self._queue_inverse(stream)
-            else:
-                L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
+            L.call("tg_dense_spd_inverse", L.ptr(self.coarse_A), self.n_coarse, stream)
         for lv in range(1, last_lv):
             c = self.levels[lv]
             if lv == 1 and self.l1_q8:
@@ -352,12 +352,15 @@ class GMG:
         self.is_setup = True
 
     def _queue_inverse(self, stream) -> None:
-        """coarse_A^-1 on the side stream once the dense matrix is assembled
-        on `stream`; its status is copied to the host behind it."""
+        """The coarsest matrix's dense assembly and inverse on the side
+        stream, once the last level's Galerkin matrices are queued on
+        `stream`; the inverse's status is copied to the host behind it."""
         self._ev_dense.record(self._torch_stream(stream))
         self._side.wait_event(self._ev_dense)
-        L.call("tg_dense_spd_inverse_async", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self._inv_info),
-               self._side.cuda_stream)
+        last, side = self.levels[-1], self._side.cuda_stream
+        L.call("tg_assemble_dense", last.grid, L.ptr(last.mats64), L.ptr(self.coarse_fidx), self.n_coarse,
+               L.ptr(self.coarse_A), side)
+        L.call("tg_dense_spd_inverse_async", L.ptr(self.coarse_A), self.n_coarse, L.ptr(self._inv_info), side)
         with torch.cuda.stream(self._side):
             self._rb_info.start(self._inv_info)
         self._ev_inv.record(self._side)
