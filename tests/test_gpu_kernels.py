@@ -532,6 +532,33 @@ def test_not_spd_is_reported():
         L.call("tg_dense_spd_inverse", L.ptr(t(B.ravel())), 2100, stream())
 
 
+def test_dense_spd_inverse_async_matches_host_synchronous():
+    """tg_dense_spd_inverse_async (the one-workgroup inverses queued without
+    a host wait, status in a device word) gives the bits of
+    tg_dense_spd_inverse for SPD matrices, the same first failing pivot + 1
+    for non-SPD ones, and rejects sizes above the one-workgroup limit."""
+    rng = np.random.default_rng(11)
+    info = torch.full((1,), -7, dtype=torch.int32, device=DEV)
+    for n in (1, 7, 33, 64, 65, 135, 140):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        A = (Q * np.logspace(0, 6, n)) @ Q.T
+        A = 0.5 * (A + A.T)
+        ref, got = t(A.ravel()), t(A.ravel())
+        L.call("tg_dense_spd_inverse", L.ptr(ref), n, stream())
+        L.call("tg_dense_spd_inverse_async", L.ptr(got), n, L.ptr(info), stream())
+        torch.cuda.synchronize()
+        assert int(h(info)[0]) == 0, n
+        assert np.array_equal(h(got), h(ref)), n
+    for n, bad in ((50, 33), (135, 0), (135, 70), (140, 139)):
+        B = np.eye(n)
+        B[bad, bad] = -1.0
+        L.call("tg_dense_spd_inverse_async", L.ptr(t(B.ravel())), n, L.ptr(info), stream())
+        torch.cuda.synchronize()
+        assert int(h(info)[0]) == bad + 1, (n, bad)
+    with pytest.raises(L.TopoHipError, match="above 140"):
+        L.call("tg_dense_spd_inverse_async", L.ptr(t(np.eye(141).ravel())), 141, L.ptr(info), stream())
+
+
 @pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 97, 135, 140, 141, 160, 161, 192, 193, 600, 1080, 2100])
 def test_dense_spd_inverse_sizes(n):
     """One-workgroup Gauss-Jordan in registers (n <= 64) and in LDS (n <=
