@@ -394,7 +394,9 @@ int tg_compliance_dd(const tg_grid* g, const double* E, const double* u, const d
  *   sequential loop. No-op once converged. Filter as in tg_stencil64.
  * oc_finalize: rho_new = clip(q/sqrt(lm_last)); *change = max|rho_new-rho|.
  * state: TG_OC_STATE_LEN device doubles {l1, l2, lm_last, done, steps,
- *   passes, ...}. */
+ *   passes, ...}; entry 27 keeps the last bisection's result across
+ *   oc_prepare calls (zero in a fresh state), around which the next first
+ *   pass evaluates its multipliers. */
 #define TG_OC_STATE_LEN 32
 int tg_oc_prepare(int64_t n, const double* rho, const double* dc, const double* dv, double move,
                   double* q, double* lo, double* hi, double* state, void* stream);

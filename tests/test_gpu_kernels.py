@@ -710,6 +710,25 @@ def test_oc_bisection_matches_oracle(dims, seed, volfrac, spread):
     got = h(rn)
     assert np.abs(got - ref).max() < 1e-12
     assert h(chg)[0] == pytest.approx(np.abs(ref - rho).max(), abs=1e-12)
+    # a second update on the same state starts its first pass around the
+    # previous result (kept in entry 27): with the crossing moved a little,
+    # far (outside the neighbours, inside the bisection's range) or not at
+    # all, the decisions stay the sequential loop's
+    assert st[27] == lm_ref
+    for scale in (1.0 + 1e-3 * rng.standard_normal(n), np.full(n, 1e4), np.ones(n)):
+        dc2 = dc * scale
+        ref2, lm2, steps2 = oracle.oc_update_filtered(rho, dc2, dv, volfrac, 0.2, H, 1.0, 0.5, True,
+                                                      return_state=True)
+        L.call("tg_oc_prepare", n, L.ptr(drho), L.ptr(t(dc2)), L.ptr(t(dv)), 0.2, L.ptr(q), L.ptr(lo), L.ptr(hi),
+               L.ptr(state), stream())
+        for _ in range(50):
+            L.call("tg_oc_bisect_step", L.grid(dims), w.size, offs.ravel().ctypes.data, w.ctypes.data, L.ptr(dhs),
+                   L.ptr(q), L.ptr(lo), L.ptr(hi), 1.0, 0.5, volfrac, 1, L.ptr(state), stream())
+        L.call("tg_oc_finalize", n, L.ptr(drho), L.ptr(q), L.ptr(lo), L.ptr(hi), L.ptr(state), L.ptr(rn),
+               L.ptr(chg), stream())
+        st = h(state)
+        assert st[3] == 1.0 and int(st[4]) == steps2 and st[2] == lm2
+        assert np.abs(h(rn) - ref2).max() < 1e-12
 
 
 def test_krylov_vector_kernels():

@@ -289,24 +289,44 @@ enum OcState {
   S_B, S_VB,       // smallest multiplier known to give volume at or below the target
   S_SA, S_SB,      // as S_A / S_B, restricted to volumes clear of the target by the margin
   S_CAND,          // OC_NPTS multipliers of the next evaluation pass
+  S_PREV = S_CAND + 15,   // the last bisection's result (kept across calls; 0: none)
   S_LEN = TG_OC_STATE_LEN
 };
 constexpr int OC_NPTS = 15;
-static_assert(S_CAND + OC_NPTS <= S_LEN, "state layout");
+static_assert(S_PREV >= S_CAND + OC_NPTS && S_PREV < S_LEN, "state layout");
 
 __global__ void k_oc_prepare(long long n, const double* __restrict__ rho, const double* __restrict__ dc,
                              const double* __restrict__ dv, double move, double* q, double* lo, double* hi,
                              double* state) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e == 0) {
-    for (int k = 0; k < S_LEN; ++k) state[k] = 0.0;
+    const double prev = state[S_PREV];
+    for (int k = 0; k < S_LEN; ++k)
+      if (k != S_PREV) state[k] = 0.0;
     state[S_L1] = 0.0;
     state[S_L2] = 1e9;
     state[S_LM] = 0.5e9;
     state[S_A] = state[S_SA] = -1.0;      // none yet
     state[S_B] = state[S_SB] = HUGE_VAL;  // none yet
-    // first pass: multipliers log-spaced over [1e-10, 1e9] (ratio 10^(19/14))
-    for (int t = 0; t < OC_NPTS; ++t) state[S_CAND + t] = pow(10.0, -10.0 + 19.0 * t / (OC_NPTS - 1));
+    if (prev > 1e-10 && prev < 1e9) {
+      // first pass around the previous iteration's multiplier, which the
+      // new crossing is usually near: it, twelve geometric neighbours from
+      // e^(+-1e-4) to e^(+-1.5) times it, and the ends of the bisection's
+      // range (a two-sided bracket whatever the crossing). Which multipliers
+      // a pass evaluates changes only how many steps it decides, not the
+      // decisions.
+      const double d[6] = {1e-4, 1e-3, 1e-2, 0.05, 0.25, 1.5};
+      state[S_CAND] = prev;
+      for (int k = 0; k < 6; ++k) {
+        state[S_CAND + 1 + 2 * k] = prev * exp(-d[k]);
+        state[S_CAND + 2 + 2 * k] = prev * exp(d[k]);
+      }
+      state[S_CAND + 13] = 1e-10;
+      state[S_CAND + 14] = 1e9;
+    } else {
+      // first pass: multipliers log-spaced over [1e-10, 1e9] (ratio 10^(19/14))
+      for (int t = 0; t < OC_NPTS; ++t) state[S_CAND + t] = pow(10.0, -10.0 + 19.0 * t / (OC_NPTS - 1));
+    }
   }
   if (e >= n) return;
   const double dcs = fmin(dc[e], -1e-12), dvs = fmax(dv[e], 1e-12);
@@ -682,7 +702,10 @@ __global__ void __launch_bounds__(256) k_oc_plan(int nb, long long n, double vol
   if (lane != 0) return;
   state[S_PASSES] += 1.0;
   state[S_DONE] = done ? 1.0 : 0.0;
-  if (done) return;
+  if (done) {
+    state[S_PREV] = lm_last;
+    return;
+  }
 
   // next pass
   double nc[OC_NPTS];
