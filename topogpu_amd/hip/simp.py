@@ -213,8 +213,9 @@ class HipSIMP:
                L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), L.ptr(self.oc_state), st)
         use_h = int(self.use_heaviside and self.beta > 1e-6)
         # each call evaluates one batch of multipliers and advances the
-        # bisection by at least two steps (usually far more); calls after
-        # convergence return at once, so five are queued first and then two
+        # bisection by at least two steps (usually far more; from the second
+        # iteration on, two or three calls finish it); calls after
+        # convergence return at once, so two are queued first and then one
         # more before each read of whether those before them converged
         def queue(k):
             for _ in range(k):
@@ -222,11 +223,11 @@ class HipSIMP:
                        L.ptr(self.hs_inv), L.ptr(self.q), L.ptr(self.lo), L.ptr(self.hi), self.beta, self.eta,
                        self.volfrac, use_h, L.ptr(self.oc_state), st)
         limit = (max_steps + 1) // 2
-        queue(5)
-        passes = 5
+        queue(2)
+        passes = 2
         while True:
             self._rb.start(self.oc_state[3:4])
-            ahead = 2 if passes < limit else 0
+            ahead = 1 if passes < limit else 0
             queue(ahead)
             if self._rb.wait()[0] or passes >= limit:
                 break
