@@ -9,6 +9,7 @@ import torch
 
 import oracle
 import topogpu_amd as tg
+from topogpu_amd.hip import lib as L
 from topogpu_amd.hip.state import HipStateSolver
 
 from gpu_util import DEV, cantilever_bc, h, rel_l2
@@ -35,6 +36,20 @@ def test_state_solve_matches_direct(gpu, precision, dims, levels):
     # warm start from the converged solution returns without a Krylov step
     st2 = s.solve()
     assert st2.iters == 0 and st2.converged
+
+
+def test_coarsest_not_spd_is_reported_by_the_solve(gpu):
+    """The coarsest inverse runs asynchronously on a side stream (108 free
+    DOFs here); a non-positive pivot still ends the solve with the library's
+    not-SPD error, read with the first scalar the solve waits for."""
+    dims = (16, 8, 8)
+    free, F, ndof = cantilever_bc(dims)
+    s = HipStateSolver(dims, free, F, n_levels=3)
+    assert s.gmg.async_inverse and 64 < s.gmg.n_coarse <= 140
+    E64 = torch.full((int(np.prod(dims)),), -1.0, dtype=torch.float64, device=DEV)   # K negative definite
+    s.set_modulus(E64, E64.float(), -1.0)
+    with pytest.raises(L.TopoHipError, match="not SPD \\(pivot 1\\)"):
+        s.solve()
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp32"])
