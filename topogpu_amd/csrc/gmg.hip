@@ -81,12 +81,16 @@ __global__ void __launch_bounds__(256) k_prolong(Grid f, Grid c, const uint8_t* 
   const unsigned t = n / (unsigned)f.NZ();
   const int k = (int)(n - t * (unsigned)f.NZ()), j = (int)(t % (unsigned)f.NY()), i = (int)(t / (unsigned)f.NY());
   const AxisW ax = axis_weights(i, c.nx), ay = axis_weights(j, c.ny), az = axis_weights(k, c.nz);
+  // the weights are 0, 1/2 and 1 and their products: exact in FP32, so the
+  // FP32 products equal the rounded FP64 ones
+  const float wx[2] = {(float)ax.wlo, (float)ax.whi}, wy[2] = {(float)ay.wlo, (float)ay.whi},
+              wz[2] = {(float)az.wlo, (float)az.whi};
   float3 v[8];
   float w[8];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const int I = (a & 4) ? ax.hi : ax.lo, J = (a & 2) ? ay.hi : ay.lo, K = (a & 1) ? az.hi : az.lo;
-    w[a] = (float)(((a & 4) ? ax.whi : ax.wlo) * ((a & 2) ? ay.whi : ay.wlo) * ((a & 1) ? az.whi : az.wlo));
+    w[a] = wx[(a >> 2) & 1] * wy[(a >> 1) & 1] * wz[a & 1];
     v[a] = ld3(ec + 3 * ((I * c.NY() + J) * c.NZ() + K));
   }
   float acc[3] = {0.f, 0.f, 0.f};
