@@ -313,7 +313,8 @@ int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, float a, floa
 /* Power-iteration step for lambda_max(D^-1 K): *vdv = sum(v*v/dinv), then
  * v <- dinv * kv (kv = K v). The Rayleigh quotient (v.Kv)/(v.Dv) is
  * scale-free, so v is not renormalised (_estimate_fine_lambda_max,
- * multigrid_v4.py:273-295). */
+ * multigrid_v4.py:273-295). vdv may be NULL: then only v is updated, with
+ * no reduction (the steps before the last, whose quotient is not read). */
 int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv,
                     void* stream);
 /* Fill helpers (graph-safe replacements for memset). */

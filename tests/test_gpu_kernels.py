@@ -751,3 +751,23 @@ def test_krylov_vector_kernels():
     rnew = b - 0.5 * a[::-1]
     assert np.allclose(h(rr_), rnew, rtol=1e-6, atol=1e-6)
     assert h(s)[2] == pytest.approx(float(rnew.astype(float) @ rnew), rel=1e-6)
+
+
+def test_power_step_with_and_without_reduction():
+    """tg_power_step32: v.Dv = sum(v*v/dinv) in FP64 and v <- dinv*kv; with
+    vdv NULL (the power iteration's steps before the last) v is updated to
+    the same bits and nothing is reduced."""
+    n = 100_003
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(n).astype(np.float32)
+    kv = rng.standard_normal(n).astype(np.float32)
+    dinv = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    s = torch.full((2,), -1.0, dtype=torch.float64, device=DEV)
+    v1, v2 = t(v), t(v)
+    L.call("tg_power_step32", n, L.ptr(v1), L.ptr(t(kv)), L.ptr(t(dinv)), L.ptr(s[0:1]), stream())
+    L.call("tg_power_step32", n, L.ptr(v2), L.ptr(t(kv)), L.ptr(t(dinv)), None, stream())
+    want = v.astype(float) ** 2 / dinv.astype(float)
+    got = h(s)
+    assert got[0] == pytest.approx(float(want.sum()), rel=1e-12)
+    assert got[1] == -1.0
+    assert np.array_equal(h(v1), dinv * kv) and np.array_equal(h(v2), h(v1))

@@ -307,17 +307,21 @@ __global__ void k_cheb_init(long long n, const float* rhs, const float* dinv, fl
   }
 }
 
+// REDUCE = false: the steps before the last, whose v.Dv nobody reads.
+template <bool REDUCE>
 __global__ void __launch_bounds__(kVecBlock) k_power_step(long long n, float* v, const float* kv, const float* dinv,
                                                          Workspace ws) {
   double acc[1] = {0.0};
   #pragma unroll 4
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const double vi = v[i];
-    acc[0] += vi * vi / (double)dinv[i];
+    if (REDUCE) {
+      const double vi = v[i];
+      acc[0] += vi * vi / (double)dinv[i];
+    }
     v[i] = dinv[i] * kv[i];
   }
-  block_partials<1>(acc, ws);
+  if (REDUCE) block_partials<1>(acc, ws);
 }
 
 __global__ void k_axpy_f32_to_f64(long long n, double* x, const float* d) {
@@ -590,9 +594,15 @@ extern "C" int tg_cheb_init32(int64_t n, const float* rhs, const float* dinv, fl
 }
 extern "C" int tg_power_step32(int64_t n, float* v, const float* kv, const float* dinv, double* vdv, void* s) {
   TG_NEED_INIT();
-  TG_REQUIRE(n > 0 && v && kv && dinv && vdv, "bad power_step args");
+  TG_REQUIRE(n > 0 && v && kv && dinv, "bad power_step args");
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_power_step, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
+  if (!vdv) {
+    hipLaunchKernelGGL(k_power_step<false>, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
+                       workspace());
+    TG_LAUNCH_CHECK();
+    return TG_OK;
+  }
+  hipLaunchKernelGGL(k_power_step<true>, dim3(nb), dim3(kVecBlock), 0, TG_STREAM(s), (long long)n, v, kv, dinv,
                      workspace());
   TG_LAUNCH_CHECK();
   return finalize_sums(1, nb, vdv, nullptr, TG_STREAM(s));

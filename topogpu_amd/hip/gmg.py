@@ -256,6 +256,9 @@ class GMG:
             self.coarse_A = torch.zeros(self.n_coarse * self.n_coarse, dtype=torch.float64, device=device)
         self.scal = torch.zeros(8, dtype=torch.float64, device=device)
         self._rb = Readback(2)
+        # (v.Kv, v.Dv) of every level's power iteration, read at once
+        self._lam_scal = torch.zeros(2 * len(self.levels), dtype=torch.float64, device=device)
+        self._rb_lam = Readback(2 * len(self.levels))
         # The one-workgroup coarsest inverse occupies one compute unit for
         # about 160 us at 135 DOFs: it runs, with the dense assembly before
         # it, on a side stream, beside the levels' stencil assembly and the
@@ -345,9 +348,13 @@ class GMG:
         prev = self._emax_at_estimate
         reestimate = np.isnan(prev) or abs(emax - prev) / max(abs(prev), 1e-12) > 0.10
         if reestimate:
+            # every level's power iteration is queued before the one read
             for lv in range(last_lv):
                 iters = self.power_iters if lv == 0 else self.coarse_power_iters
-                self.levels[lv].lam = self.estimate_lambda(lv, iters, stream)
+                self._queue_lambda(lv, iters, stream, self._lam_scal[2 * lv:2 * lv + 2])
+            q = self._rb_lam.read(self._lam_scal[:2 * last_lv])
+            for lv in range(last_lv):
+                self.levels[lv].lam = self._quotient(q[2 * lv], q[2 * lv + 1])
             self._emax_at_estimate = emax
         self.is_setup = True
 
@@ -396,18 +403,28 @@ class GMG:
         density the all-ones vector is almost a rigid translation, has almost
         no high-frequency content and 20 steps then underestimate lambda_max
         badly on large grids."""
+        self._queue_lambda(lv, iters, stream, self.scal[0:2])
+        return self._quotient(*self._rb.read(self.scal[0:2]))
+
+    def _queue_lambda(self, lv: int, iters: int, stream, s) -> None:
+        """Queue estimate_lambda's power iteration on level lv; the last
+        step's v.Kv and v.Dv land in s[0] and s[1]. It uses the level's x
+        and r as v and Kv."""
         Lv = self.levels[lv]
         v, kv = Lv.x[:Lv.n], Lv.r[:Lv.n]
         v.copy_(Lv.v0)
-        s = self.scal
-        ep = L.ep(L.TG_EP_DOT, dot=s[0:1])
+        store, ep = L.ep(L.TG_EP_STORE), L.ep(L.TG_EP_DOT, dot=s[0:1])
         # the Rayleigh quotient is scale-free and v grows by about lambda_max
-        # per step, so the steps run back to back and only the last quotient
-        # is read
-        for _ in range(iters):
-            self._launch(lv, self.op(lv, accurate=True), v, kv, ep, stream, nrhs=1)
-            L.call("tg_power_step32", Lv.n, L.ptr(v), L.ptr(kv), L.ptr(Lv.dinv), L.ptr(s[1:2]), stream)
-        vkv, vdv = self._rb.read(s[0:2])
+        # per step, so the steps run back to back and only the last step
+        # reduces: its quotient is the one read
+        for k in range(iters):
+            last = k == iters - 1
+            self._launch(lv, self.op(lv, accurate=True), v, kv, ep if last else store, stream, nrhs=1)
+            L.call("tg_power_step32", Lv.n, L.ptr(v), L.ptr(kv), L.ptr(Lv.dinv), L.ptr(s[1:2]) if last else None,
+                   stream)
+
+    @staticmethod
+    def _quotient(vkv: float, vdv: float) -> float:
         lam = vkv / vdv if vdv > 1e-300 and np.isfinite(vkv / vdv) else 1.0
         return max(lam, 1e-6)
 
