@@ -50,6 +50,12 @@ typedef struct {
  * (src/gpu_fem/fem_gpu.py:361-368) and FusedMatvec.__init__ KE packing
  * (src/gpu_fem/cuda_fused_matvec.py:279-292, 316-330). */
 int tg_init(const double* ke_unit_host);
+/* *dev = the device address of pinned host memory (hipHostMalloc) at host,
+ * so that a kernel writes a result the host polls for straight into it, with
+ * no copy launch in between (the Krylov loop's r.r). TG_ERR_ARG when host is
+ * not such memory. No counterpart in the reference, whose host reads go
+ * through CuPy's device-to-host copies (.get() / float()). */
+int tg_mapped_pointer(const void* host, void** dev);
 const char* tg_last_error(void);
 int tg_version(void);
 
@@ -286,7 +292,8 @@ int tg_dot32(int64_t n, const float* a, const float* b, const float* c, double* 
              double* out2, void* stream);
 int tg_dot64(int64_t n, const double* a, const double* b, double* out, void* stream);
 /* PCG step: alpha = *rz / *pq; x += alpha p; r -= alpha q; *rr = sum(r*r).
- * Replaces the vector updates of _cupy_pcg (solver_v2.py:606-611). */
+ * Replaces the vector updates of _cupy_pcg (solver_v2.py:606-611). rr may be
+ * the tg_mapped_pointer address of pinned host memory. */
 int tg_pcg_xr(int64_t n, float* x, float* r, const float* p, const float* q, const double* rz,
               const double* pq, double* rr, void* stream);
 /* Flexible (Polak-Ribiere) direction: beta = -(*zq) / (*pq); p = z + beta p.

@@ -771,3 +771,30 @@ def test_power_step_with_and_without_reduction():
     assert got[0] == pytest.approx(float(want.sum()), rel=1e-12)
     assert got[1] == -1.0
     assert np.array_equal(h(v1), dinv * kv) and np.array_equal(h(v2), h(v1))
+
+
+def test_pcg_update_writes_rr_into_mapped_host_memory():
+    """tg_pcg_xr_b with r.r written through tg_mapped_pointer into pinned
+    host memory gives the bits of the device-scalar output, readable on the
+    host once an event behind the kernel has passed (the Krylov loop's
+    r.r read); pageable host memory is refused by tg_mapped_pointer."""
+    from topogpu_amd.hip.readback import Mapped
+    n, nr = 50_001, 3
+    rng = np.random.default_rng(9)
+    vecs = [rng.standard_normal(nr * n).astype(np.float32) for _ in range(4)]
+    s = torch.zeros(3, nr, dtype=torch.float64, device=DEV)
+    s[0], s[1] = torch.tensor([2.0, 3.0, 5.0]), torch.tensor([4.0, 1.0, 7.0])
+    outs = []
+    for mapped in (False, True):
+        x, r, p, q = (t(v) for v in vecs)
+        m = Mapped(nr) if mapped else None
+        rr = m.device_ptr if mapped else L.ptr(s[2])
+        L.call("tg_pcg_xr_b", n, nr, L.ptr(x), L.ptr(r), L.ptr(p), L.ptr(q), L.ptr(s[0]), L.ptr(s[1]), rr,
+               stream())
+        outs.append((m.start().wait() if mapped else h(s[2]), h(x), h(r)))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert all(np.array_equal(a, b) for a, b in zip(outs[0][1:], outs[1][1:]))
+    want = [float(np.sum(outs[0][2][k * n:(k + 1) * n].astype(float) ** 2)) for k in range(nr)]
+    assert outs[1][0] == pytest.approx(want, rel=1e-12)
+    with pytest.raises(L.TopoHipError, match="pinned host memory"):
+        L.call("tg_mapped_pointer", np.zeros(16).ctypes.data, C.addressof(C.c_void_p()))

@@ -385,6 +385,18 @@ extern "C" int tg_normalize64(int64_t n, const double* x, const double* ss, doub
 extern "C" const char* tg_last_error(void) { return g_err; }
 extern "C" int tg_version(void) { return 1; }
 
+extern "C" int tg_mapped_pointer(const void* host, void** dev) {
+  TG_REQUIRE(host && dev, "bad mapped_pointer args");
+  void* d = nullptr;
+  const hipError_t e = hipHostGetDevicePointer(&d, const_cast<void*>(host), 0);
+  if (e != hipSuccess || !d) {
+    (void)hipGetLastError();
+    return ::tg::set_error(TG_ERR_ARG, "not pinned host memory mapped into the device: %s", hipGetErrorString(e));
+  }
+  *dev = d;
+  return TG_OK;
+}
+
 extern "C" int tg_init(const double* ke) {
   TG_REQUIRE(ke != nullptr, "ke_unit_host is NULL");
   const int dev = current_device();

@@ -48,6 +48,7 @@ def ep(mode, rhs=None, dinv=None, d=None, x=None, r=None, a1=0.0, a2=0.0, a3=0.0
 
 _SIGS = {
     "tg_init": [_vp],
+    "tg_mapped_pointer": [_vp, _vp],
     "tg_last_error": None,
     "tg_version": [],
     "tg_apply32": [C.POINTER(TgOp32), _vp, _vp, C.POINTER(TgEp32), _vp],

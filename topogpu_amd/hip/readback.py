@@ -11,13 +11,24 @@ well before the value is needed (the host keeps queueing work meanwhile).
 
 from __future__ import annotations
 
+import ctypes as C
 import time
 
 import numpy as np
 import torch
 
+from . import lib as L
+
 # a wait longer than this is not on a latency-critical path: block
 POLL_S = 2e-3
+
+
+def _poll(event: torch.cuda.Event) -> None:
+    deadline = time.perf_counter() + POLL_S
+    while not event.query():
+        if time.perf_counter() > deadline:
+            event.synchronize()
+            return
 
 
 class Readback:
@@ -42,12 +53,33 @@ class Readback:
 
     def wait(self) -> np.ndarray:
         """The values of the last ``start``, once its copies have finished."""
-        deadline = time.perf_counter() + POLL_S
-        while not self.event.query():
-            if time.perf_counter() > deadline:
-                self.event.synchronize()
-                break
+        _poll(self.event)
         return self.host[:self._k].numpy().copy()
 
     def read(self, *srcs: torch.Tensor) -> np.ndarray:
         return self.start(*srcs).wait()
+
+
+class Mapped:
+    """n values that a kernel writes straight into pinned host memory
+    through its device address (``device_ptr``, an output argument of the
+    kernel's entry point). ``start`` records an event behind that kernel and
+    ``wait`` returns the values once the event has passed: no copy launch
+    sits between the kernel and the host's read."""
+
+    def __init__(self, n: int, dtype=torch.float64):
+        self.host = torch.zeros(int(n), dtype=dtype, pin_memory=True)
+        self.event = torch.cuda.Event()
+        dev = C.c_void_p()
+        L.call("tg_mapped_pointer", self.host.data_ptr(), C.addressof(dev))
+        self.device_ptr: int = dev.value
+
+    def start(self) -> "Mapped":
+        """Record the event on torch's current stream, behind the kernel
+        that writes the values."""
+        self.event.record()
+        return self
+
+    def wait(self) -> np.ndarray:
+        _poll(self.event)
+        return self.host.numpy().copy()

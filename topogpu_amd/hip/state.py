@@ -31,7 +31,7 @@ from .. import fem
 from . import graph
 from . import lib as L
 from .gmg import GMG, launch_op
-from .readback import Readback
+from .readback import Mapped, Readback
 
 # precision -> (Krylov operator, level-0 smoother operator, level-0
 #               post-correction operator, level-1 post-correction operator,
@@ -151,11 +151,13 @@ class HipStateSolver:
         self.p = torch.zeros(nb, **f32)
         self.q = torch.zeros(nb, **f32)
         # device scalars, one row of nrhs per quantity:
-        # 0 rz, 1 pq, 2 rr(inner), 3 zq, 4 rr(outer), 5 compliance
+        # 0 rz, 1 pq, 2 (unused), 3 zq, 4 rr(outer), 5 compliance
         self.s = torch.zeros(8, self.nrhs, dtype=torch.float64, device=self.dev)
         # host reads of device scalars through pinned memory: r.r in the
         # Krylov loop, the other residual norms and sums, the compliances
-        self._rb_rr = Readback(self.nrhs)
+        # r.r of the Krylov loop: written by the update kernel straight into
+        # pinned host memory (row 2 of s stays unused)
+        self._rb_rr = Mapped(self.nrhs)
         self._rb = Readback(max(8 * self.nrhs, int(restart) + 2))
         self._rb_c = Readback(self.nrhs)
         self.E64 = None
@@ -392,7 +394,7 @@ class HipStateSolver:
         V-cycle's first part (gmg.begin, which touches only the hierarchy's
         buffers) is queued behind the copy before the host waits, so the GPU
         is not idle while the host reads the value and queues the rest."""
-        self._rb_rr.start(self.s[2])
+        self._rb_rr.start()
         if speculate:
             self._begin(st)
         rr = self._rb_rr.wait()
@@ -425,7 +427,7 @@ class HipStateSolver:
         ep = L.ep(L.TG_EP_DOT, dot=s[1])
         launch_op(self.gmg.timer, 0, self.fine_op(), L.ptr(self.p), L.ptr(self.q), ep, st, n, self.n_elem, nr)
         L.call("tg_pcg_xr_b", n, nr, L.ptr(self.d), L.ptr(self.r), L.ptr(self.p), L.ptr(self.q),
-               L.ptr(s[0]), L.ptr(s[1]), L.ptr(s[2]), st)
+               L.ptr(s[0]), L.ptr(s[1]), self._rb_rr.device_ptr, st)
 
     def _replaying(self) -> bool:
         """Graph replay is on and no timer wants per-launch events."""
