@@ -52,8 +52,10 @@ typedef struct {
 int tg_init(const double* ke_unit_host);
 /* *dev = the device address of pinned host memory (hipHostMalloc) at host,
  * so that a kernel writes a result the host polls for straight into it, with
- * no copy launch in between (the Krylov loop's r.r). TG_ERR_ARG when host is
- * not such memory. No counterpart in the reference, whose host reads go
+ * no copy launch in between. Any reduced scalar output of the entry points
+ * below (the Krylov loop's r.r, the FP64 residual norms, dot products) may
+ * be such an address, and so may a scalar input (tg_normalize64's ss).
+ * TG_ERR_ARG when host is not such memory. No counterpart in the reference, whose host reads go
  * through CuPy's device-to-host copies (.get() / float()). */
 int tg_mapped_pointer(const void* host, void** dev);
 const char* tg_last_error(void);

@@ -151,7 +151,8 @@ class HipStateSolver:
         self.p = torch.zeros(nb, **f32)
         self.q = torch.zeros(nb, **f32)
         # device scalars, one row of nrhs per quantity:
-        # 0 rz, 1 pq, 2 (unused), 3 zq, 4 rr(outer), 5 compliance
+        # 0 rz, 1 pq, 3 zq (rows 2, 4 and 5 unused: r.r, the FP64
+        # residual norms and the compliances go to mapped host memory)
         self.s = torch.zeros(8, self.nrhs, dtype=torch.float64, device=self.dev)
         # host reads of device scalars through pinned memory: r.r in the
         # Krylov loop, the other residual norms and sums, the compliances
@@ -159,7 +160,8 @@ class HipStateSolver:
         # pinned host memory (row 2 of s stays unused)
         self._rb_rr = Mapped(self.nrhs)
         self._rb = Readback(max(8 * self.nrhs, int(restart) + 2))
-        self._rb_c = Readback(self.nrhs)
+        self._rb_res = Mapped(self.nrhs)
+        self._rb_c = Mapped(self.nrhs)
         self.E64 = None
         self.E32 = None
         self.last: SolveStats | None = None
@@ -194,11 +196,11 @@ class HipStateSolver:
         the new r is queued before the host waits for the norms (the next
         call of _iteration may then pass begun=True)."""
         L.call("tg_apply64_b", self.grid, L.ptr(self.mask), L.ptr(self.E64), self.nrhs, L.ptr(self.u),
-               L.ptr(self.F), L.ptr(self.r64), L.ptr(self.r), L.ptr(self.s[4]), self.stream)
-        self._rb.start(self.s[4])
+               L.ptr(self.F), L.ptr(self.r64), L.ptr(self.r), self._rb_res.device_ptr, self.stream)
+        self._rb_res.start()
         if speculate:
             self._begin(self.stream)
-        rr = self._rb.wait()
+        rr = self._rb_res.wait()
         self.gmg.check(block=False)
         return np.sqrt(np.maximum(rr, 0.0))
 
@@ -333,14 +335,14 @@ class HipStateSolver:
         total = cycles = 0
         converged = False
         while total < self.max_krylov:
-            beta = float(self.residual64()[0])            # r64 = F - K u, s[4] = r.r
+            beta = float(self.residual64()[0])            # r64 = F - K u and its norm
             cycles += 1
             if not np.isfinite(beta):
                 raise FloatingPointError("non-finite residual in FGMRES")
             if beta / b_norm <= tol:
                 converged = True
                 break
-            L.call("tg_normalize64", n, L.ptr(self.r64), L.ptr(self.s[4]), L.ptr(V[0]), L.ptr(self.r), st)
+            L.call("tg_normalize64", n, L.ptr(self.r64), self._rb_res.device_ptr, L.ptr(V[0]), L.ptr(self.r), st)
             H = np.zeros((m + 1, m))
             cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
             g[0] = beta
@@ -464,12 +466,12 @@ class HipStateSolver:
             self._replay("begin", st)
         self._replay("finish_first" if first else "finish", st)
 
-    def compliances_start(self) -> Readback:
-        """Queue F_k . u_k in FP64 for every load case (solver_v4.py:244) and
-        its copy to the host; the returned reader's wait() gives the values,
-        so the host may queue further work before it blocks."""
-        L.call("tg_dot64_b", self.n, self.nrhs, L.ptr(self.F), L.ptr(self.u), L.ptr(self.s[5]), self.stream)
-        return self._rb_c.start(self.s[5])
+    def compliances_start(self) -> Mapped:
+        """Queue F_k . u_k in FP64 for every load case (solver_v4.py:244),
+        written into mapped host memory; the returned reader's wait() gives
+        the values, so the host may queue further work before it blocks."""
+        L.call("tg_dot64_b", self.n, self.nrhs, L.ptr(self.F), L.ptr(self.u), self._rb_c.device_ptr, self.stream)
+        return self._rb_c.start()
 
     def compliances(self) -> np.ndarray:
         """F_k . u_k in FP64 for every load case (solver_v4.py:244)."""
